@@ -1,0 +1,191 @@
+"""Benchmark of the volumetric-cloud path on one MI355X.
+
+One step = one frame of the 1920x1080 scenario workload: every noise texture regenerated on the GPU
+(weather 512^2, curl 128^2, low-res 128^3, hi-res 32^3), then the full-frame cloud march (all
+pixels). Inputs live in HBM before timing starts. Prints one JSON line with ms/frame, density
+samples/s (the value), the march kernel's roofline position and a CPU baseline (the FP32 oracle
+on this host's cores, on a bounded sample of the same frame).
+"""
+import argparse
+import glob
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "project-marshmallow_amd")
+METRIC = "ms/frame @1920x1080 + density samples/s; surrogate vs plain at ≤1/255 err"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic texel-footprint bytes (4 weather texels + 8 low-res texels per density sample,
+# 4 curl texels + 8 hi-res texels per detail sample), per format set, plus 16 B of output per pixel.
+BYTES_PER_SAMPLE = {
+    "fast": {"density": 4 * 2 + 8 * 4, "detail": 4 * 4 + 8 * 2},
+    "ref": {"density": 4 * 4 + 8 * 4, "detail": 4 * 4 + 8 * 4},
+    "prec": {"density": 4 * 2 + 8 * 4, "detail": 4 * 8 + 8 * 8},
+}
+
+
+def load_package():
+    if "marshmallow_amd" in sys.modules:
+        return sys.modules["marshmallow_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "marshmallow_amd", os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["marshmallow_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def traffic_from_profiles(fset, width, height):
+    """HBM bytes per march launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-kernel
+    calibrated), if a summary for this workload is committed under profiles/."""
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*pmc*.json"))):
+        try:
+            with open(path) as f:
+                d = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if d.get("format_set") == fset and d.get("width") == width and d.get("height") == height:
+            return d.get("hbm_bytes_per_launch")
+    return None
+
+
+def cpu_baseline(mm, ns, cam, sun, sky, prm, W, H, budget_s):
+    """FP32 oracle (all host threads) on whole 1/16 pixel phases of the same frame until the budget
+    is spent; returns density samples/s."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    orc = oracle.Oracle()
+    ref = mm.NoiseSet("ref")
+    ref.generate(mm.noise_params(seed=1, weather_seed=1, coverage=0.5), mm.current_stream())
+    import torch
+
+    torch.cuda.synchronize()
+    t = {k: getattr(ref, k).tensor.cpu().numpy() for k in ("weather", "curl", "low_res", "hi_res")}
+    tex = orc.textures(t["weather"].reshape(512, 512, 4), t["curl"].reshape(128, 128, 4),
+                       t["low_res"].reshape(128, 128, 128, 4), t["hi_res"].reshape(32, 32, 32, 4))
+    order = [0, 5, 10, 15, 2, 7, 8, 13, 1, 4, 11, 14, 3, 6, 9, 12]
+    samples, pixels, elapsed, done = 0, 0, 0.0, 0
+    for p in order:
+        t0 = time.perf_counter()
+        _, cnt = orc.march(cam, sun, sky, tex, prm, W, H, phase=p, counts=True)
+        elapsed += time.perf_counter() - t0
+        samples += int(cnt[..., 0].sum())
+        pixels += len(range(p % 4, W, 4)) * len(range(p // 4, H, 4))
+        done += 1
+        if elapsed >= budget_s:
+            break
+    threads = int(orc.L.orc_max_threads())
+    return {"value": samples / elapsed, "unit": "density samples/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 oracle march of {done}/16 pixel phases of the same 1920x1080 frame "
+                      f"({pixels} px, {samples} density samples, {elapsed:.1f} s)",
+            "ms_per_frame_projected": None if not samples else None}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--fset", default="fast", choices=["fast", "ref", "prec"])
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the frame shards by row bands "
+              "or views with no exchange; see the sharding note in the design doc)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    mm = load_package()
+    torch.cuda.set_device(0)
+    W, H = args.width, args.height
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(altitude=200.0, pitch_deg=20.0, fov_deg=60.0, aspect=W / H,
+                                                    sun_elevation_deg=25.0, time=0.0))
+    prm = mm.march_params("scenario")
+    ns = mm.NoiseSet(args.fset)
+    nparams = mm.noise_params(seed=1, weather_seed=1, coverage=0.5)
+    out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
+    stream = mm.current_stream()
+
+    # instrumented run (outside the timed region): sample counts of this frame
+    ns.generate(nparams, stream)
+    counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+    mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream, counts=counts)
+    torch.cuda.synchronize()
+    n_density, n_detail = (int(v) for v in counts.cpu())
+
+    for _ in range(args.warmup):
+        ns.generate(nparams, stream)
+        mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream)
+    torch.cuda.synchronize()
+
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for i in range(args.steps):
+        ev[i][0].record()
+        ns.generate(nparams, stream)
+        ev[i][1].record()
+        mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream)
+        ev[i][2].record()
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = start.elapsed_time(end) / args.steps
+    noise_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps
+    march_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps
+    assert torch.isfinite(out).all().item()
+
+    b = BYTES_PER_SAMPLE[args.fset]
+    alg_bytes = n_density * b["density"] + n_detail * b["detail"] + W * H * 16
+    achieved = alg_bytes / (march_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(args.fset, W, H),
+                "kernel": "march_kernel", "kernel_ms": round(march_ms, 4),
+                "algorithmic_bytes_per_launch": alg_bytes,
+                "bytes_per_density_sample": b["density"], "bytes_per_detail_sample": b["detail"]}
+    rec = {
+        "metric": METRIC,
+        "value": round(n_density / (ms_step * 1e-3), 1),
+        "unit": "density samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded procedural noise volumes and weather map, regenerated on the GPU every step",
+        "config": {"workload": f"{W}x{H} scenario frame: all noise textures regenerated, then full-frame FP32 march "
+                               f"(Earth radius 6.371e6 m, shell 1500-4000 m, camera 200 m pitched +20 deg, 60 deg "
+                               f"FOV, sun 25 deg, coverage 0.5, t = 0, weather seed 1)",
+                   "width": W, "height": H, "format_set": args.fset, "coarse_steps_max": 128, "fine_steps_max": 256},
+        "ms_per_frame": round(ms_step, 4),
+        "frames_per_s": round(1000.0 / ms_step, 2),
+        "march_ms": round(march_ms, 4),
+        "noise_ms": round(noise_ms, 4),
+        "density_samples_per_frame": n_density,
+        "detail_samples_per_frame": n_detail,
+        "wall_s": round(wall, 3),
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if args.cpu_budget > 0:
+        cb = cpu_baseline(mm, ns, cam, sun, sky, prm, W, H, args.cpu_budget)
+        cb["ms_per_frame_projected"] = round(n_density / cb["value"] * 1e3, 1)
+        rec["cpu_baseline"] = cb
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

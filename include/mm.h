@@ -1,0 +1,252 @@
+/*
+ * mm.h - C ABI of the MI355X volumetric-cloud path (noise generation + cloud raymarch).
+ *
+ * Drop-in for the reference's compute dispatch of compute-clouds.comp:
+ *   - the operator it replaces is ComputeShader (SkyEngine/SkyEngine/Shader.h:286-377), whose
+ *     pipeline layout is {storage image, storage image, UBOs + samplers}
+ *     (SkyEngine/SkyEngine/Shader.cpp:662-693, 911-954);
+ *   - the uniform blocks below are byte-identical to the std140 blocks the shader declares
+ *     (SkyEngine/SkyEngine/Shaders/compute-clouds.comp:12-41) and the host structs that fill them
+ *     (SkyEngine/SkyEngine/Shader.h:24-28, SkyEngine/SkyEngine/SkyManager.h:8-14, 28-36);
+ *   - the output is the RGBA32F storage image the shader writes with imageStore
+ *     (compute-clouds.comp:9, 499): row-major, 4 floats per pixel, row 0 = top of the view.
+ *
+ * Every entry point returns MM_OK (0) or a negative MM_E_* code and never throws.
+ * Device pointers are caller-owned; work is enqueued on the given stream (a hipStream_t passed as
+ * void*, NULL = the null stream) and is asynchronous: the caller synchronises.
+ */
+#ifndef MM_H
+#define MM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MM_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- errors */
+enum {
+    MM_OK = 0,
+    MM_E_NULL = -1,      /* a required pointer is NULL */
+    MM_E_DIM = -2,       /* zero/negative/unsupported image or texture dimensions */
+    MM_E_PHASE = -3,     /* phase outside [-2, 15] or sun.color.a outside [0, 16) */
+    MM_E_PITCH = -4,     /* row pitch smaller than the width */
+    MM_E_FORMAT = -5,    /* unknown or unsupported texture format / layout */
+    MM_E_PARAM = -6,     /* non-finite uniforms, zero-length sun direction, bad params */
+    MM_E_DEVICE = -7,    /* HIP runtime error or no device */
+    MM_E_SIZE = -8,      /* a texture allocation is smaller than its format requires */
+    MM_E_VERSION = -9    /* params.size does not match this library's struct */
+};
+const char* mm_strerror(int code);
+int mm_abi_version(void);
+
+/* ---------------------------------------------------------------- uniform blocks (std140) */
+/* UniformCameraObject: Shader.h:24-28, compute-clouds.comp:12-17. 160 bytes.
+ * view and proj are column-major 4x4 (glm/GLSL order: m[col*4 + row]).
+ * cameraParams = (aspect, tan(0.5 * DEG2RAD * fov), 0, 0) (VulkanApplication.cpp:365-366). */
+typedef struct mm_camera_ubo {
+    float view[16];
+    float proj[16];
+    float cameraPosition[4];
+    float cameraParams[4];
+} mm_camera_ubo;
+
+/* UniformSunObject: SkyManager.h:8-14, compute-clouds.comp:26-32. 116 bytes used, padded to 128.
+ * color[3] carries the 1/16 pixel phase (VulkanApplication.cpp:381-384).
+ * directionBasis columns: 0 = N, 1 = sun direction, 2 = B (SkyManager.cpp:60-69). */
+typedef struct mm_sun_ubo {
+    float location[4];
+    float direction[4];
+    float color[4];
+    float directionBasis[16];
+    float intensity;
+    float _pad[3];
+} mm_sun_ubo;
+
+/* UniformSkyObject: SkyManager.h:28-36, compute-clouds.comp:35-41. 52 bytes used, padded to 64.
+ * wind.xyz = wind direction, wind.w = time. */
+typedef struct mm_sky_ubo {
+    float betaR[4];
+    float betaV[4];
+    float wind[4];
+    float mie_directional;
+    float _pad[3];
+} mm_sky_ubo;
+
+/* ---------------------------------------------------------------- textures */
+/* Texel formats. The reference binds every texture as RGBA8 UNORM (Texture.h:29, 85).
+ * Channel use by the march: weather R = coverage, B = cloud type (comp:237, 245; G unused);
+ * curl .xyz (comp:218); low-res R = base shape, GBA = erosion FBMs (comp:238-247);
+ * hi-res RGB = erosion FBMs (comp:223-224). */
+enum {
+    MM_FMT_RGBA8 = 1,       /* 4 x UNORM8, 4 B/texel: the reference format, every texture */
+    MM_FMT_RG8 = 2,         /* 2 x UNORM8, 2 B/texel: weather only, (R = coverage, G = type) */
+    MM_FMT_R8E16 = 3,       /* low-res only, 4 B/texel: byte0 = R UNORM8, bytes 2-3 = 5g+2b+a (0..2040) */
+    MM_FMT_E16 = 4,         /* hi-res only, 2 B/texel: 5r+2g+b (0..2040), the 0.625/0.25/0.125 erosion sum */
+    MM_FMT_FP8X4 = 5,       /* low-res only, 4 x FP8 e4m3 with per-channel FP32 scale (mm_texture.scale) */
+    MM_FMT_FP16X4 = 6       /* 4 x FP16, 8 B/texel: curl (x,y,z,0) or hi-res (r,g,b,0) */
+};
+
+/* Memory layouts. */
+enum {
+    MM_LAYOUT_LINEAR = 0,   /* reference: z = slice, rows top-down, x fastest (Texture.cpp:502-538) */
+    MM_LAYOUT_APRON = 1     /* (N+1) per axis, index N repeats index 0 so every filter footprint is
+                               contiguous in x: REPEAT wrap without index wrapping in the kernel */
+};
+
+typedef struct mm_texture {
+    void* data;             /* device pointer */
+    size_t bytes;           /* bytes available at data (validated against format and dims) */
+    int32_t width, height, depth;  /* depth = 1 for 2D textures */
+    int32_t format;         /* MM_FMT_* */
+    int32_t layout;         /* MM_LAYOUT_* */
+    int32_t _pad;
+    float scale[4];         /* per-channel dequantisation scale (FP8 formats); ignored otherwise */
+} mm_texture;
+
+/* The four textures compute-clouds.comp samples (set 2, bindings 4, 6, 7, 8: Shader.cpp:771-908). */
+typedef struct mm_noise_set {
+    mm_texture weather;     /* binding 4 cloudPlacement, 2D */
+    mm_texture curl;        /* binding 6 curlNoise, 2D */
+    mm_texture low_res;     /* binding 7 lowResCloudShape, 3D */
+    mm_texture hi_res;      /* binding 8 hiResCloudShape, 3D */
+} mm_noise_set;
+
+/* Bytes a texture of the given shape, format and layout occupies (0 if unsupported). */
+size_t mm_texture_bytes(int32_t width, int32_t height, int32_t depth, int32_t format, int32_t layout);
+
+/* ---------------------------------------------------------------- noise generation */
+enum {
+    MM_GEN_WEATHER = 1, MM_GEN_CURL = 2, MM_GEN_LOW_RES = 4, MM_GEN_HI_RES = 8, MM_GEN_ALL = 15
+};
+
+typedef struct mm_noise_params {
+    uint32_t size;          /* sizeof(mm_noise_params) */
+    uint32_t what;          /* MM_GEN_* mask */
+    uint32_t seed;          /* seeds the low-res and hi-res volumes */
+    uint32_t weather_seed;  /* seeds the weather map */
+    float coverage;         /* 0..1: shifts the weather coverage (R) field; 0.5 = unbiased */
+    int32_t _reserved[7];
+} mm_noise_params;
+
+void mm_noise_params_default(mm_noise_params* p);
+
+/* Generate the textures selected by p->what into out (dims, format and layout are read from
+ * out->*; data must be allocated by the caller with at least mm_texture_bytes()).
+ * workspace: device scratch of at least mm_noise_workspace_bytes() bytes (used by the curl
+ * generator's gradient table, float curl field and global min/max), or NULL to let the library
+ * allocate it stream-ordered for this call.
+ * The curl map restates ImageUtils.cpp:176-223 (GenerateCurlNoise) and does not depend on the seed.
+ * Replaces: the baked TGA/PNG assets loaded by VulkanApplication.cpp:253-262 and the offline
+ * curl tool GenerateCurlNoise (ImageUtils.h:12). */
+size_t mm_noise_workspace_bytes(const mm_noise_set* out);
+int mm_generate_noise(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream);
+
+/* Repack a linear RGBA8 texture (the reference's asset layout) into another format/layout.
+ * kind selects the channel mapping: 0 = weather, 1 = curl, 2 = low-res, 3 = hi-res. */
+int mm_pack_texture(int32_t kind, const mm_texture* src, const mm_texture* dst, void* stream);
+
+/* ---------------------------------------------------------------- march */
+enum {
+    MM_MARCH_ISECT_FP64 = 1u << 0,   /* ray/shell intersection in FP64 (same formula as comp:147-177) */
+    MM_MARCH_BOUNDED = 1u << 1,      /* every fine-mode sample counts toward max_fine_steps (the
+                                        reference's `continue` at comp:437 does not count) */
+    MM_MARCH_NO_COMPACT = 1u << 2    /* disable wave-level compaction of terminated rays */
+};
+
+enum { MM_CONE_PLAIN = 0, MM_CONE_SURROGATE = 1 };
+
+typedef struct mm_march_params {
+    uint32_t size;              /* sizeof(mm_march_params) */
+    uint32_t flags;             /* MM_MARCH_* */
+    /* shell: spheres are given by their diameter like the reference's vec4(center, w) (comp:359-361) */
+    float inner_diameter;       /* ATMOSPHERE_RADIUS = 2e6 */
+    float outer_diameter;       /* ATMOSPHERE_RADIUS * 1.02 */
+    float center_y;             /* earthCenter.y = -ATMOSPHERE_RADIUS * 0.5 * 0.995 (comp:358) */
+    float thickness;            /* 0.5 * ATMOSPHERE_RADIUS * 0.02 (comp:360) */
+    float proj_radius;          /* 0.5 * ATMOSPHERE_RADIUS (getProjectedShellPoint, comp:181) */
+    /* stepping (comp:390, 426-433, 468-481) */
+    float coarse_step_frac;     /* 0.05: coarse step = frac * thickness */
+    float fine_step_mul;        /* 0.3 */
+    int32_t miss_limit;         /* 10 */
+    float opacity_cutoff;       /* 0.99 */
+    int32_t max_steps;          /* 100: break when ++steps > max_steps (counted steps); < 0 = off */
+    int32_t max_coarse_steps;   /* counted coarse steps limit, same `++n > limit` form; < 0 = off */
+    int32_t max_fine_steps;     /* fine steps limit; < 0 = off */
+    /* sampling (comp:218, 223, 236, 238, 279, 414) */
+    float wind_strength;        /* 20 */
+    float weather_scale;        /* 0.000009 */
+    float low_scale;            /* 0.00002 */
+    float curl_scale;           /* 0.0001 */
+    float hi_scale;             /* 0.0004 */
+    int32_t subtexel_bits;      /* sampler weight precision: 8 = fixed-point like the hardware, 0 = float */
+    /* light cone */
+    int32_t cone_mode;          /* MM_CONE_* */
+    float surrogate_threshold;  /* predicted-error bound above which the plain cone is used */
+    const void* surrogate;      /* device pointer to packed surrogate weights (cone_mode 1) */
+    int32_t _reserved[8];
+} mm_march_params;
+
+/* Preset with the reference's constants (compute-clouds.comp, bit-for-bit). */
+void mm_march_params_reference(mm_march_params* p);
+/* Preset of the scenario configs: Earth radius 6.371e6 m, shell 1500-4000 m, <=128 coarse and
+ * <=256 fine steps, FP64 intersection. */
+void mm_march_params_scenario(mm_march_params* p);
+
+#define MM_PHASE_ALL (-1)       /* march every pixel */
+#define MM_PHASE_FROM_UBO (-2)  /* phase = int(sun.color.a) like the reference (comp:292) */
+
+/* One cloud dispatch. Replaces ComputeShader::bindShader + vkCmdDispatch of compute-clouds.comp
+ * (Shader.h:358-376, VulkanApplication.cpp:1061-1071; the shader body is comp:288-500).
+ * out_rgba32f: device, row-major with row_pitch_px pixels per row, 4 floats per pixel.
+ * phase 0..15 marches pixels (4i + phase%4, 4j + phase/4); other pixels are left untouched.
+ * night_sky: binding 5 (RGBA8 linear 2D), may be NULL (black night sky).
+ * params: NULL = mm_march_params_reference. */
+int mm_march_clouds(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                    const mm_noise_set* textures, const mm_texture* night_sky,
+                    float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
+                    int32_t phase, const mm_march_params* params, void* stream);
+
+/* Per-ray counters of the last launch of the instrumented march (density samples = cloudTest
+ * evaluations, detail samples = cloudHiRes evaluations). counts: device int64[2] accumulators. */
+int mm_march_clouds_counted(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                            const mm_noise_set* textures, const mm_texture* night_sky,
+                            float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
+                            int32_t phase, const mm_march_params* params,
+                            unsigned long long* counts, void* stream);
+
+/* ---------------------------------------------------------------- uniform builder (caller side) */
+/* Scene description from which the three UBOs are derived exactly as VulkanApplication.cpp:351-395
+ * does through Camera (camera.cpp:27-39, camera.h:72) and SkyManager (SkyManager.cpp:15-70). */
+typedef struct mm_scene {
+    float camera_pos[3];        /* world position */
+    float pitch;                /* Camera::m_pitch (radians); the view looks along -forward */
+    float yaw;                  /* Camera::m_yaw (radians) */
+    float fov_deg;              /* vertical field of view in degrees (Camera::m_fov) */
+    float aspect;               /* width / height */
+    float sun_elevation;        /* SkyManager elevation parameter (SkyManager.cpp:46-54) */
+    float sun_azimuth;          /* SkyManager azimuth parameter */
+    float time;                 /* sky.wind.w */
+    float wind[3];              /* sky.wind.xyz, reference default (1, 0.05, 1) */
+    float rayleigh;             /* 2 */
+    float mie;                  /* 0.005 */
+    float mie_directional;      /* 0.8 */
+    float turbidity;            /* uninitialised in the reference; 10 here */
+    int32_t phase;              /* written to sun.color.a */
+} mm_scene;
+
+void mm_scene_default(mm_scene* s);
+/* Scene of the scenario configs: camera at `altitude` m, view pitched up `pitch_deg` and facing -z,
+ * `fov_deg` vertical FOV, sun `sun_elevation_deg` above the horizon towards -z. */
+void mm_scene_scenario(mm_scene* s, float altitude, float pitch_deg, float fov_deg, float aspect,
+                       float sun_elevation_deg, float time);
+int mm_build_ubos(const mm_scene* s, mm_camera_ubo* camera, mm_sun_ubo* sun, mm_sky_ubo* sky);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MM_H */

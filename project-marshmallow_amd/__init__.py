@@ -1,0 +1,298 @@
+"""Host-side mirror of the reference's cloud compute operator over the C ABI of libmm.so.
+
+The reference binds compute-clouds.comp through ``ComputeShader`` (SkyEngine/SkyEngine/Shader.h:286-377):
+``updateUniformBuffers(cam, camPrev, sky, sun)`` copies the uniform blocks (Shader.cpp:967-992) and
+``bindShader`` + ``vkCmdDispatch`` record one 1/16-pixel march into a ping-ponged RGBA32F storage
+image (Shader.h:358-376, VulkanApplication.cpp:1061-1071). :class:`ComputeShader` below keeps those
+names and that behaviour on top of ``mm_march_clouds``; the uniform producers ``SkyManager`` /
+``Camera`` (SkyManager.cpp, camera.cpp) are reached through ``mm_build_ubos``.
+
+Device memory and streams come from PyTorch (plumbing only); every computation runs in libmm.so.
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmm.so")
+
+# ---------------------------------------------------------------- constants (include/mm.h)
+MM_OK = 0
+ERRORS = {
+    -1: "MM_E_NULL", -2: "MM_E_DIM", -3: "MM_E_PHASE", -4: "MM_E_PITCH", -5: "MM_E_FORMAT",
+    -6: "MM_E_PARAM", -7: "MM_E_DEVICE", -8: "MM_E_SIZE", -9: "MM_E_VERSION",
+}
+MM_E_NULL, MM_E_DIM, MM_E_PHASE, MM_E_PITCH, MM_E_FORMAT, MM_E_PARAM, MM_E_DEVICE, MM_E_SIZE, MM_E_VERSION = range(-1, -10, -1)
+
+FMT_RGBA8, FMT_RG8, FMT_R8E16, FMT_E16, FMT_FP8X4, FMT_FP16X4 = 1, 2, 3, 4, 5, 6
+LAYOUT_LINEAR, LAYOUT_APRON = 0, 1
+GEN_WEATHER, GEN_CURL, GEN_LOW_RES, GEN_HI_RES, GEN_ALL = 1, 2, 4, 8, 15
+MARCH_ISECT_FP64, MARCH_BOUNDED, MARCH_NO_COMPACT = 1, 2, 4
+CONE_PLAIN, CONE_SURROGATE = 0, 1
+PHASE_ALL, PHASE_FROM_UBO = -1, -2
+TEXEL_BYTES = {FMT_RGBA8: 4, FMT_RG8: 2, FMT_R8E16: 4, FMT_E16: 2, FMT_FP8X4: 4, FMT_FP16X4: 8}
+
+# Texel format sets the march kernel runs on: (weather, curl, low-res, hi-res) formats + layout.
+FORMAT_SETS = {
+    "ref": ((FMT_RGBA8, FMT_RGBA8, FMT_RGBA8, FMT_RGBA8), LAYOUT_LINEAR),
+    "fast": ((FMT_RG8, FMT_RGBA8, FMT_R8E16, FMT_E16), LAYOUT_APRON),
+    "prec": ((FMT_RG8, FMT_FP16X4, FMT_FP8X4, FMT_FP16X4), LAYOUT_APRON),
+}
+
+
+class MMError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: {ERRORS.get(code, code)} ({code})")
+        self.code = code
+
+
+# ---------------------------------------------------------------- ABI structs
+class CameraUBO(ctypes.Structure):
+    _fields_ = [("view", ctypes.c_float * 16), ("proj", ctypes.c_float * 16),
+                ("cameraPosition", ctypes.c_float * 4), ("cameraParams", ctypes.c_float * 4)]
+
+
+class SunUBO(ctypes.Structure):
+    _fields_ = [("location", ctypes.c_float * 4), ("direction", ctypes.c_float * 4), ("color", ctypes.c_float * 4),
+                ("directionBasis", ctypes.c_float * 16), ("intensity", ctypes.c_float), ("_pad", ctypes.c_float * 3)]
+
+
+class SkyUBO(ctypes.Structure):
+    _fields_ = [("betaR", ctypes.c_float * 4), ("betaV", ctypes.c_float * 4), ("wind", ctypes.c_float * 4),
+                ("mie_directional", ctypes.c_float), ("_pad", ctypes.c_float * 3)]
+
+
+class Texture(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("bytes", ctypes.c_size_t), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("depth", ctypes.c_int32), ("format", ctypes.c_int32),
+                ("layout", ctypes.c_int32), ("_pad", ctypes.c_int32), ("scale", ctypes.c_float * 4)]
+
+
+class NoiseSetC(ctypes.Structure):
+    _fields_ = [("weather", Texture), ("curl", Texture), ("low_res", Texture), ("hi_res", Texture)]
+
+
+class NoiseParams(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("what", ctypes.c_uint32), ("seed", ctypes.c_uint32),
+                ("weather_seed", ctypes.c_uint32), ("coverage", ctypes.c_float), ("_reserved", ctypes.c_int32 * 7)]
+
+
+class MarchParams(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("inner_diameter", ctypes.c_float), ("outer_diameter", ctypes.c_float), ("center_y", ctypes.c_float),
+                ("thickness", ctypes.c_float), ("proj_radius", ctypes.c_float),
+                ("coarse_step_frac", ctypes.c_float), ("fine_step_mul", ctypes.c_float), ("miss_limit", ctypes.c_int32),
+                ("opacity_cutoff", ctypes.c_float), ("max_steps", ctypes.c_int32),
+                ("max_coarse_steps", ctypes.c_int32), ("max_fine_steps", ctypes.c_int32),
+                ("wind_strength", ctypes.c_float), ("weather_scale", ctypes.c_float), ("low_scale", ctypes.c_float),
+                ("curl_scale", ctypes.c_float), ("hi_scale", ctypes.c_float), ("subtexel_bits", ctypes.c_int32),
+                ("cone_mode", ctypes.c_int32), ("surrogate_threshold", ctypes.c_float),
+                ("surrogate", ctypes.c_void_p), ("_reserved", ctypes.c_int32 * 8)]
+
+
+class Scene(ctypes.Structure):
+    _fields_ = [("camera_pos", ctypes.c_float * 3), ("pitch", ctypes.c_float), ("yaw", ctypes.c_float),
+                ("fov_deg", ctypes.c_float), ("aspect", ctypes.c_float), ("sun_elevation", ctypes.c_float),
+                ("sun_azimuth", ctypes.c_float), ("time", ctypes.c_float), ("wind", ctypes.c_float * 3),
+                ("rayleigh", ctypes.c_float), ("mie", ctypes.c_float), ("mie_directional", ctypes.c_float),
+                ("turbidity", ctypes.c_float), ("phase", ctypes.c_int32)]
+
+
+# Every symbol include/mm.h declares, with its ctypes signature.
+_P = ctypes.POINTER
+_SIGS = {
+    "mm_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+    "mm_abi_version": (ctypes.c_int, []),
+    "mm_texture_bytes": (ctypes.c_size_t, [ctypes.c_int32] * 5),
+    "mm_noise_params_default": (None, [_P(NoiseParams)]),
+    "mm_noise_workspace_bytes": (ctypes.c_size_t, [_P(NoiseSetC)]),
+    "mm_generate_noise": (ctypes.c_int, [_P(NoiseParams), _P(NoiseSetC), ctypes.c_void_p, ctypes.c_void_p]),
+    "mm_pack_texture": (ctypes.c_int, [ctypes.c_int32, _P(Texture), _P(Texture), ctypes.c_void_p]),
+    "mm_march_params_reference": (None, [_P(MarchParams)]),
+    "mm_march_params_scenario": (None, [_P(MarchParams)]),
+    "mm_march_clouds": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
+                                       ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       _P(MarchParams), ctypes.c_void_p]),
+    "mm_march_clouds_counted": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
+                                               ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32, _P(MarchParams), ctypes.c_void_p, ctypes.c_void_p]),
+    "mm_scene_default": (None, [_P(Scene)]),
+    "mm_scene_scenario": (None, [_P(Scene)] + [ctypes.c_float] * 6),
+    "mm_build_ubos": (ctypes.c_int, [_P(Scene), _P(CameraUBO), _P(SunUBO), _P(SkyUBO)]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libmm.so (built in-tree by ``make`` in this directory). Raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libmm.so not built: {LIB_PATH} (run __graft_entry__.build())")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(code: int, what: str) -> None:
+    if code != MM_OK:
+        raise MMError(code, what)
+
+
+# ---------------------------------------------------------------- presets / builders (host only)
+def march_params(preset: str = "reference", **overrides) -> MarchParams:
+    p = MarchParams()
+    if preset == "reference":
+        lib().mm_march_params_reference(ctypes.byref(p))
+    elif preset == "scenario":
+        lib().mm_march_params_scenario(ctypes.byref(p))
+    else:
+        raise ValueError(preset)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def noise_params(seed: int = 1, weather_seed: int = 1, coverage: float = 0.5, what: int = GEN_ALL) -> NoiseParams:
+    p = NoiseParams()
+    lib().mm_noise_params_default(ctypes.byref(p))
+    p.seed, p.weather_seed, p.coverage, p.what = seed, weather_seed, coverage, what
+    return p
+
+
+def scenario_scene(altitude=200.0, pitch_deg=20.0, fov_deg=60.0, aspect=1.0, sun_elevation_deg=25.0, time=0.0,
+                   phase=0) -> Scene:
+    s = Scene()
+    lib().mm_scene_scenario(ctypes.byref(s), altitude, pitch_deg, fov_deg, aspect, sun_elevation_deg, time)
+    s.phase = phase
+    return s
+
+
+def build_ubos(scene: Scene):
+    cam, sun, sky = CameraUBO(), SunUBO(), SkyUBO()
+    check(lib().mm_build_ubos(ctypes.byref(scene), ctypes.byref(cam), ctypes.byref(sun), ctypes.byref(sky)),
+          "mm_build_ubos")
+    return cam, sun, sky
+
+
+def texture_bytes(w: int, h: int, d: int, fmt: int, layout: int) -> int:
+    return int(lib().mm_texture_bytes(w, h, d, fmt, layout))
+
+
+# ---------------------------------------------------------------- device textures
+@dataclass
+class DeviceTexture:
+    tensor: object          # torch.uint8 device tensor holding the texels
+    desc: Texture
+
+
+def device_texture(w: int, h: int, d: int, fmt: int, layout: int, device="cuda", scale=(1.0, 1.0, 1.0, 1.0),
+                   fill: int | None = None) -> DeviceTexture:
+    import torch
+
+    n = texture_bytes(w, h, d, fmt, layout)
+    if n == 0:
+        raise MMError(MM_E_FORMAT, "texture_bytes")
+    t = torch.empty(n, dtype=torch.uint8, device=device)
+    if fill is not None:
+        t.fill_(fill)
+    desc = Texture(t.data_ptr(), n, w, h, d, fmt, layout, 0, (ctypes.c_float * 4)(*scale))
+    return DeviceTexture(t, desc)
+
+
+class NoiseSet:
+    """The four march textures (bindings 4, 6, 7, 8 of compute-clouds.comp) in one format set."""
+
+    def __init__(self, fset: str = "fast", weather_dim=512, curl_dim=128, low_dim=128, hi_dim=32, device="cuda",
+                 fp8_scale=1.0 / 448.0):
+        fmts, layout = FORMAT_SETS[fset]
+        self.fset = fset
+        low_scale = (fp8_scale,) * 4 if fmts[2] == FMT_FP8X4 else (1.0,) * 4
+        self.weather = device_texture(weather_dim, weather_dim, 1, fmts[0], layout, device)
+        self.curl = device_texture(curl_dim, curl_dim, 1, fmts[1], layout, device)
+        self.low_res = device_texture(low_dim, low_dim, low_dim, fmts[2], layout, device, scale=low_scale)
+        self.hi_res = device_texture(hi_dim, hi_dim, hi_dim, fmts[3], layout, device)
+        self.c = NoiseSetC(self.weather.desc, self.curl.desc, self.low_res.desc, self.hi_res.desc)
+        self.workspace = None
+
+    def generate(self, params: NoiseParams, stream: int = 0, workspace=True) -> None:
+        import torch
+
+        ws = None
+        if workspace:
+            if self.workspace is None:
+                nbytes = int(lib().mm_noise_workspace_bytes(ctypes.byref(self.c)))
+                self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=self.weather.tensor.device)
+            ws = self.workspace.data_ptr()
+        check(lib().mm_generate_noise(ctypes.byref(params), ctypes.byref(self.c), ws, stream), "mm_generate_noise")
+
+
+def current_stream() -> int:
+    import torch
+
+    return torch.cuda.current_stream().cuda_stream
+
+
+def march_clouds(cam: CameraUBO, sun: SunUBO, sky: SkyUBO, noise: NoiseSet, out, width: int, height: int,
+                 phase: int = PHASE_ALL, params: MarchParams | None = None, night: DeviceTexture | None = None,
+                 pitch: int | None = None, stream: int | None = None, counts=None) -> None:
+    """One cloud dispatch into ``out`` (a float32 device tensor of >= height*pitch*4 elements)."""
+    pitch = width if pitch is None else pitch
+    stream = current_stream() if stream is None else stream
+    pp = ctypes.byref(params) if params is not None else None
+    nt = ctypes.byref(night.desc) if night is not None else None
+    if counts is None:
+        code = lib().mm_march_clouds(ctypes.byref(cam), ctypes.byref(sun), ctypes.byref(sky), ctypes.byref(noise.c), nt,
+                                     out.data_ptr(), width, height, pitch, phase, pp, stream)
+        check(code, "mm_march_clouds")
+    else:
+        code = lib().mm_march_clouds_counted(ctypes.byref(cam), ctypes.byref(sun), ctypes.byref(sky),
+                                             ctypes.byref(noise.c), nt, out.data_ptr(), width, height, pitch, phase,
+                                             pp, counts.data_ptr(), stream)
+        check(code, "mm_march_clouds_counted")
+
+
+class ComputeShader:
+    """Mirror of the reference's ComputeShader operator (Shader.h:286-377) for compute-clouds.comp.
+
+    Holds the two ping-ponged RGBA32F storage images (storageBufferSetA/B) and the uniform copies;
+    ``bind_shader`` swaps the images like Shader.h:358-376, ``dispatch`` records the march of the
+    pixel phase carried in ``sun.color.a`` (comp:292) into the current target image.
+    """
+
+    def __init__(self, width: int, height: int, noise: NoiseSet, night: DeviceTexture | None = None,
+                 params: MarchParams | None = None, device="cuda"):
+        import torch
+
+        self.width, self.height = width, height
+        self.noise, self.night, self.params = noise, night, params
+        self.storage_a = torch.zeros(height * width * 4, dtype=torch.float32, device=device)
+        self.storage_b = torch.zeros(height * width * 4, dtype=torch.float32, device=device)
+        self.swapped_buffers = False
+        self.cam, self.cam_prev, self.sky, self.sun = CameraUBO(), CameraUBO(), SkyUBO(), SunUBO()
+
+    def update_uniform_buffers(self, cam: CameraUBO, cam_prev: CameraUBO, sky: SkyUBO, sun: SunUBO) -> None:
+        # Shader.cpp:967-992 memcpy's each block into its host-coherent UBO.
+        ctypes.memmove(ctypes.byref(self.cam), ctypes.byref(cam), ctypes.sizeof(CameraUBO))
+        ctypes.memmove(ctypes.byref(self.cam_prev), ctypes.byref(cam_prev), ctypes.sizeof(CameraUBO))
+        ctypes.memmove(ctypes.byref(self.sky), ctypes.byref(sky), ctypes.sizeof(SkyUBO))
+        ctypes.memmove(ctypes.byref(self.sun), ctypes.byref(sun), ctypes.sizeof(SunUBO))
+
+    def bind_shader(self):
+        target = self.storage_b if self.swapped_buffers else self.storage_a
+        self.swapped_buffers = not self.swapped_buffers
+        return target
+
+    def dispatch(self, stream: int | None = None):
+        target = self.bind_shader()
+        march_clouds(self.cam, self.sun, self.sky, self.noise, target, self.width, self.height,
+                     phase=PHASE_FROM_UBO, params=self.params, night=self.night, stream=stream)
+        return target

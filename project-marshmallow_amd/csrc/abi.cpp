@@ -1,0 +1,389 @@
+// C ABI of the cloud path (include/mm.h): argument validation, presets, hoisting of the
+// uniform-derived per-dispatch constants, and dispatch to the HIP launchers. No exception crosses
+// the boundary; every failure is a negative MM_E_* code.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include "mm_internal.hpp"
+
+namespace {
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+bool finite_n(const float* v, int n) {
+    for (int i = 0; i < n; i++)
+        if (!isfinite(v[i])) return false;
+    return true;
+}
+
+int texel_bytes(int fmt) {
+    switch (fmt) {
+        case MM_FMT_RGBA8: return 4;
+        case MM_FMT_RG8: return 2;
+        case MM_FMT_R8E16: return 4;
+        case MM_FMT_E16: return 2;
+        case MM_FMT_FP8X4: return 4;
+        case MM_FMT_FP16X4: return 8;
+    }
+    return 0;
+}
+
+// Which formats each texture slot accepts (generation, packing and march).
+bool slot_format_ok(int slot, int fmt) {
+    switch (slot) {
+        case 0: return fmt == MM_FMT_RGBA8 || fmt == MM_FMT_RG8;                              // weather
+        case 1: return fmt == MM_FMT_RGBA8 || fmt == MM_FMT_FP16X4;                           // curl
+        case 2: return fmt == MM_FMT_RGBA8 || fmt == MM_FMT_R8E16 || fmt == MM_FMT_FP8X4;     // low-res
+        case 3: return fmt == MM_FMT_RGBA8 || fmt == MM_FMT_E16 || fmt == MM_FMT_FP16X4;      // hi-res
+    }
+    return false;
+}
+
+// Shape/format/size validation of one texture slot. dims = 2 or 3.
+int check_texture(const mm_texture* t, int slot, int dims, bool need_pow2) {
+    if (!t->data) return MM_E_NULL;
+    if (t->layout != MM_LAYOUT_LINEAR && t->layout != MM_LAYOUT_APRON) return MM_E_FORMAT;
+    if (!slot_format_ok(slot, t->format)) return MM_E_FORMAT;
+    if (t->width <= 0 || t->height <= 0 || t->depth <= 0) return MM_E_DIM;
+    if (dims == 2 && t->depth != 1) return MM_E_DIM;
+    if (dims == 3 && (t->width != t->height || t->width != t->depth)) return MM_E_DIM;
+    if (need_pow2 && (!is_pow2(t->width) || !is_pow2(t->height))) return MM_E_DIM;
+    if (t->width > 8192 || t->height > 8192 || t->depth > 1024) return MM_E_DIM;
+    if (t->format == MM_FMT_FP8X4) {
+        for (int k = 0; k < 4; k++)
+            if (!(t->scale[k] > 0.f) || !isfinite(t->scale[k])) return MM_E_PARAM;
+    }
+    size_t need = mm_texture_bytes(t->width, t->height, t->depth, t->format, t->layout);
+    if (need == 0) return MM_E_FORMAT;
+    if (t->bytes < need) return MM_E_SIZE;
+    return MM_OK;
+}
+
+// Format sets the march kernel is instantiated for.
+int format_set(const mm_noise_set* n) {
+    const mm_texture &w = n->weather, &c = n->curl, &l = n->low_res, &h = n->hi_res;
+    auto all = [&](int layout) {
+        return w.layout == layout && c.layout == layout && l.layout == layout && h.layout == layout;
+    };
+    if (all(MM_LAYOUT_LINEAR) && w.format == MM_FMT_RGBA8 && c.format == MM_FMT_RGBA8 && l.format == MM_FMT_RGBA8 &&
+        h.format == MM_FMT_RGBA8)
+        return mm::FSET_REF;
+    if (all(MM_LAYOUT_APRON) && w.format == MM_FMT_RG8 && c.format == MM_FMT_RGBA8 && l.format == MM_FMT_R8E16 &&
+        h.format == MM_FMT_E16)
+        return mm::FSET_FAST;
+    if (all(MM_LAYOUT_APRON) && w.format == MM_FMT_RG8 && c.format == MM_FMT_FP16X4 && l.format == MM_FMT_FP8X4 &&
+        h.format == MM_FMT_FP16X4)
+        return mm::FSET_PREC;
+    return -1;
+}
+
+mm::TexView view_of(const mm_texture& t) {
+    mm::TexView v;
+    v.data = (const uint8_t*)t.data;
+    v.w = t.width; v.h = t.height; v.d = t.depth;
+    const int a = t.layout == MM_LAYOUT_APRON ? 1 : 0;
+    v.stride = t.width + a;
+    v.sliceStride = (t.width + a) * (t.height + a);
+    for (int k = 0; k < 4; k++) v.scale[k] = t.scale[k];
+    return v;
+}
+
+int check_params(const mm_march_params* p) {
+    if (p->size != sizeof(mm_march_params)) return MM_E_VERSION;
+    const float f[] = {p->inner_diameter, p->outer_diameter, p->center_y, p->thickness, p->proj_radius,
+                       p->coarse_step_frac, p->fine_step_mul, p->opacity_cutoff, p->wind_strength, p->weather_scale,
+                       p->low_scale, p->curl_scale, p->hi_scale};
+    if (!finite_n(f, (int)(sizeof(f) / sizeof(f[0])))) return MM_E_PARAM;
+    if (!(p->inner_diameter > 0.f) || !(p->outer_diameter > 0.f) || !(p->thickness > 0.f)) return MM_E_PARAM;
+    if (!(p->coarse_step_frac > 0.f) || !(p->fine_step_mul > 0.f) || p->fine_step_mul > 1.f) return MM_E_PARAM;
+    if (p->miss_limit < 1) return MM_E_PARAM;
+    if (p->subtexel_bits != 0 && p->subtexel_bits != 8) return MM_E_PARAM;
+    if (p->cone_mode != MM_CONE_PLAIN) return MM_E_PARAM;
+    return MM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mm_strerror(int code) {
+    switch (code) {
+        case MM_OK: return "ok";
+        case MM_E_NULL: return "required pointer is NULL";
+        case MM_E_DIM: return "unsupported image or texture dimensions";
+        case MM_E_PHASE: return "phase out of range";
+        case MM_E_PITCH: return "row pitch smaller than width";
+        case MM_E_FORMAT: return "unsupported texture format or layout";
+        case MM_E_PARAM: return "invalid or non-finite parameters";
+        case MM_E_DEVICE: return "HIP runtime error";
+        case MM_E_SIZE: return "texture allocation too small";
+        case MM_E_VERSION: return "struct size mismatch";
+    }
+    return "unknown error";
+}
+
+int mm_abi_version(void) { return MM_ABI_VERSION; }
+
+size_t mm_texture_bytes(int32_t width, int32_t height, int32_t depth, int32_t format, int32_t layout) {
+    const int tb = texel_bytes(format);
+    if (tb == 0 || width <= 0 || height <= 0 || depth <= 0) return 0;
+    if (layout != MM_LAYOUT_LINEAR && layout != MM_LAYOUT_APRON) return 0;
+    const size_t a = layout == MM_LAYOUT_APRON ? 1 : 0;
+    const size_t d = depth > 1 ? (size_t)depth + a : 1;
+    return (size_t)tb * ((size_t)width + a) * ((size_t)height + a) * d;
+}
+
+void mm_noise_params_default(mm_noise_params* p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->size = sizeof(mm_noise_params);
+    p->what = MM_GEN_ALL;
+    p->seed = 1;
+    p->weather_seed = 1;
+    p->coverage = 0.5f;
+}
+
+void mm_march_params_reference(mm_march_params* p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->size = sizeof(mm_march_params);
+    p->flags = 0;
+    const float AR = 2000000.0f;                 // ATMOSPHERE_RADIUS (comp:56)
+    p->inner_diameter = AR;
+    p->outer_diameter = AR * 1.02f;              // comp:361
+    p->center_y = -AR * 0.5f * 0.995f;           // comp:358
+    p->thickness = 0.5f * AR * 0.02f;            // comp:360
+    p->proj_radius = 0.5f * AR;                  // comp:181
+    p->coarse_step_frac = 0.05f;                 // comp:390
+    p->fine_step_mul = 0.3f;                     // comp:431, 472
+    p->miss_limit = 10;                          // comp:470
+    p->opacity_cutoff = 0.99f;                   // comp:476
+    p->max_steps = 100;                          // comp:286, 481
+    p->max_coarse_steps = -1;
+    p->max_fine_steps = -1;
+    p->wind_strength = 20.0f;                    // comp:279
+    p->weather_scale = 0.000009f;                // comp:236
+    p->low_scale = 0.00002f;                     // comp:238
+    p->curl_scale = 0.0001f;                     // comp:217
+    p->hi_scale = 0.0004f;                       // comp:223
+    p->subtexel_bits = 8;
+    p->cone_mode = MM_CONE_PLAIN;
+    p->surrogate_threshold = 0.f;
+    p->surrogate = nullptr;
+}
+
+void mm_march_params_scenario(mm_march_params* p) {
+    if (!p) return;
+    mm_march_params_reference(p);
+    const float R = 6371000.0f;  // Earth radius
+    p->inner_diameter = 2.0f * (R + 1500.0f);
+    p->outer_diameter = 2.0f * (R + 4000.0f);
+    p->center_y = -R;
+    p->thickness = 2500.0f;
+    p->proj_radius = R + 1500.0f;
+    p->flags = MM_MARCH_ISECT_FP64 | MM_MARCH_BOUNDED;
+    p->max_steps = -1;
+    p->max_coarse_steps = 128;
+    p->max_fine_steps = 256;
+}
+
+size_t mm_noise_workspace_bytes(const mm_noise_set* out) {
+    (void)out;
+    return mm::noise_workspace_bytes();
+}
+
+int mm_generate_noise(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream) {
+    if (!p || !out) return MM_E_NULL;
+    if (p->size != sizeof(mm_noise_params)) return MM_E_VERSION;
+    if ((p->what & ~(uint32_t)MM_GEN_ALL) != 0) return MM_E_PARAM;
+    if (!isfinite(p->coverage) || p->coverage < 0.f || p->coverage > 1.f) return MM_E_PARAM;
+    int e;
+    if ((p->what & MM_GEN_WEATHER) && (e = check_texture(&out->weather, 0, 2, false))) return e;
+    if ((p->what & MM_GEN_CURL) && (e = check_texture(&out->curl, 1, 2, false))) return e;
+    if ((p->what & MM_GEN_LOW_RES) && (e = check_texture(&out->low_res, 2, 3, false))) return e;
+    if ((p->what & MM_GEN_HI_RES) && (e = check_texture(&out->hi_res, 3, 3, false))) return e;
+    if (p->what & MM_GEN_CURL) {
+        if (out->curl.width != out->curl.height || out->curl.width > 512) return MM_E_DIM;
+    }
+    if (p->what & MM_GEN_LOW_RES) {
+        if (out->low_res.format == MM_FMT_R8E16 && out->low_res.layout != MM_LAYOUT_APRON &&
+            out->low_res.layout != MM_LAYOUT_LINEAR)
+            return MM_E_FORMAT;
+    }
+    void* ws = workspace;
+    hipStream_t s = (hipStream_t)stream;
+    bool own = false;
+    if (!ws && (p->what & MM_GEN_CURL)) {
+        if (hipMallocAsync(&ws, mm::noise_workspace_bytes(), s) != hipSuccess) return MM_E_DEVICE;
+        own = true;
+    }
+    int r = mm::launch_generate(p, out, ws, stream);
+    if (own) (void)hipFreeAsync(ws, s);
+    return r;
+}
+
+int mm_pack_texture(int32_t kind, const mm_texture* src, const mm_texture* dst, void* stream) {
+    if (!src || !dst) return MM_E_NULL;
+    if (kind < 0 || kind > 3) return MM_E_PARAM;
+    const int dims = kind >= 2 ? 3 : 2;
+    int e;
+    if ((e = check_texture(src, kind, dims, false))) return e;
+    if (src->format != MM_FMT_RGBA8 || src->layout != MM_LAYOUT_LINEAR) return MM_E_FORMAT;
+    if ((e = check_texture(dst, kind, dims, false))) return e;
+    if (dst->width != src->width || dst->height != src->height || dst->depth != src->depth) return MM_E_DIM;
+    return mm::launch_pack(kind, src, dst, stream);
+}
+
+static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                      const mm_noise_set* tex, const mm_texture* night_sky, float* out, int32_t width, int32_t height,
+                      int32_t pitch, int32_t phase, const mm_march_params* params, unsigned long long* counts,
+                      void* stream) {
+    if (!camera || !sun || !sky || !tex || !out) return MM_E_NULL;
+    if (width <= 0 || height <= 0 || width > 32768 || height > 32768) return MM_E_DIM;
+    if (pitch < width) return MM_E_PITCH;
+    mm_march_params ref;
+    if (!params) {
+        mm_march_params_reference(&ref);
+        params = &ref;
+    }
+    int e = check_params(params);
+    if (e) return e;
+    if (phase == MM_PHASE_FROM_UBO) {
+        float a = sun->color[3];
+        if (!(a >= 0.f && a < 16.f)) return MM_E_PHASE;
+        phase = (int)a;  // GLSL int() truncates (comp:292)
+    }
+    if (phase < -1 || phase > 15) return MM_E_PHASE;
+    if (!finite_n(camera->view, 16) || !finite_n(camera->cameraPosition, 4) || !finite_n(camera->cameraParams, 2))
+        return MM_E_PARAM;
+    if (!finite_n(sun->direction, 4) || !finite_n(sun->color, 3) || !finite_n(sun->directionBasis, 16) ||
+        !isfinite(sun->intensity))
+        return MM_E_PARAM;
+    if (!finite_n(sky->betaR, 3) || !finite_n(sky->betaV, 3) || !finite_n(sky->wind, 4) ||
+        !isfinite(sky->mie_directional))
+        return MM_E_PARAM;
+    const float* B = sun->directionBasis;
+    if (B[4] * B[4] + B[5] * B[5] + B[6] * B[6] == 0.f) return MM_E_PARAM;
+    if ((e = check_texture(&tex->weather, 0, 2, true))) return e;
+    if ((e = check_texture(&tex->curl, 1, 2, true))) return e;
+    if ((e = check_texture(&tex->low_res, 2, 3, true))) return e;
+    if ((e = check_texture(&tex->hi_res, 3, 3, true))) return e;
+    const int fset = format_set(tex);
+    if (fset < 0) return MM_E_FORMAT;
+    if (night_sky) {
+        if (!night_sky->data) return MM_E_NULL;
+        if (night_sky->format != MM_FMT_RGBA8 || night_sky->layout != MM_LAYOUT_LINEAR) return MM_E_FORMAT;
+        if (night_sky->depth != 1 || night_sky->width <= 0 || night_sky->height <= 0) return MM_E_DIM;
+        if (night_sky->bytes < mm_texture_bytes(night_sky->width, night_sky->height, 1, MM_FMT_RGBA8, MM_LAYOUT_LINEAR))
+            return MM_E_SIZE;
+    }
+
+    mm::MarchConsts c;
+    memset(&c, 0, sizeof(c));
+    const float* V = camera->view;
+    for (int k = 0; k < 3; k++) {
+        c.camPos[k] = camera->cameraPosition[k];
+        c.camLook[k] = V[4 * k + 2];   // (view[0][2], view[1][2], view[2][2]) (comp:312)
+        c.camRight[k] = V[4 * k + 0];  // comp:313
+        c.camUp[k] = V[4 * k + 1];     // comp:314
+        c.refPoint[k] = c.camPos[k] - c.camLook[k];
+    }
+    c.aspect = camera->cameraParams[0];
+    c.tanH = camera->cameraParams[1];
+    {
+        float x = B[4], y = B[5], z = B[6];
+        float s = 1.0f / sqrtf((x * x + y * y) + z * z);
+        c.sunDir[0] = x * s; c.sunDir[1] = y * s; c.sunDir[2] = z * s;
+    }
+    const float cone[6][3] = {{0, 0.6f, 0}, {0, 0.5f, 0.05f}, {0.1f, 0.75f, 0}, {0.2f, 2.5f, 0.3f}, {0, 6, 0},
+                              {-0.1f, 1, -0.2f}};
+    for (int i = 0; i < 6; i++)
+        for (int k = 0; k < 3; k++)
+            c.coneOffsets[i][k] = ((B[k] * cone[i][0]) + (B[4 + k] * cone[i][1])) + (B[8 + k] * cone[i][2]);
+    c.earthCenter[0] = c.camPos[0];
+    c.earthCenter[1] = params->center_y;
+    c.earthCenter[2] = c.camPos[2];
+    c.innerDiam = params->inner_diameter;
+    c.outerDiam = params->outer_diameter;
+    c.thickness = params->thickness;
+    c.invThickness = 1.0f / params->thickness;
+    c.projRadius = params->proj_radius;
+    c.stepSize0 = params->coarse_step_frac * params->thickness;
+    c.fineMul = params->fine_step_mul;
+    for (int k = 0; k < 3; k++) {
+        c.betaR[k] = sky->betaR[k];
+        c.betaM[k] = sky->betaV[k];
+        c.sunColor[k] = sun->color[k];
+        c.wind[k] = sky->wind[k];
+    }
+    c.sunIntensity = sun->intensity;
+    c.sunDirY = sun->direction[1];
+    c.mieG = sky->mie_directional;
+    c.time = sky->wind[3];
+    c.windStrength = params->wind_strength;
+    c.weatherScale = params->weather_scale;
+    c.lowScale = params->low_scale;
+    c.curlScale = params->curl_scale;
+    c.hiScale = params->hi_scale;
+    c.opacityCutoff = params->opacity_cutoff;
+    {
+        // fromAngleAxis(normalize(vec3(1, 0, 1)), sun.direction.y * 0.5) (comp:256-277, 369)
+        float s = 1.0f / sqrtf((1.0f * 1.0f + 0.0f * 0.0f) + 1.0f * 1.0f);
+        float ax = 1.0f * s, ay = 0.0f * s, az = 1.0f * s;
+        float ang = sun->direction[1] * 0.5f;
+        float ct = cosf(ang), st = sinf(ang), omc = 1.f - ct;
+        float m[3][3] = {{ct + ax * ax * omc, ay * ax * omc + az * st, az * ax * omc - ay * st},
+                         {ax * ay * omc - az * st, ct + ay * ay * omc, az * ay * omc + ax * st},
+                         {ax * az * omc + ay * st, ay * az * omc - ax * st, ct + az * az * omc}};
+        memcpy(c.nightRot, m, sizeof(m));
+    }
+    c.missLimit = params->miss_limit;
+    c.maxSteps = params->max_steps;
+    c.maxCoarse = params->max_coarse_steps;
+    c.maxFine = params->max_fine_steps;
+    c.flags = (int)params->flags;
+    c.subtexelBits = params->subtexel_bits;
+    c.width = width;
+    c.height = height;
+    c.pitch = pitch;
+    c.phase = phase;
+    if (phase >= 0) {
+        c.gridW = (width - (phase & 3) + 3) / 4;
+        c.gridH = (height - (phase >> 2) + 3) / 4;
+    } else {
+        c.gridW = width;
+        c.gridH = height;
+    }
+    if (c.gridW <= 0 || c.gridH <= 0) return MM_OK;  // no pixel of this phase in the image
+    c.tilesX = (c.gridW + 15) / 16;
+
+    mm::MarchTextures t;
+    memset(&t, 0, sizeof(t));
+    t.weather = view_of(tex->weather);
+    t.curl = view_of(tex->curl);
+    t.low = view_of(tex->low_res);
+    t.hi = view_of(tex->hi_res);
+    if (night_sky) {
+        t.night = view_of(*night_sky);
+        t.hasNight = 1;
+    }
+    return mm::launch_march(fset, c, t, out, counts, stream);
+}
+
+int mm_march_clouds(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                    const mm_noise_set* textures, const mm_texture* night_sky, float* out_rgba32f, int32_t width,
+                    int32_t height, int32_t row_pitch_px, int32_t phase, const mm_march_params* params, void* stream) {
+    return march_impl(camera, sun, sky, textures, night_sky, out_rgba32f, width, height, row_pitch_px, phase, params,
+                      nullptr, stream);
+}
+
+int mm_march_clouds_counted(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                            const mm_noise_set* textures, const mm_texture* night_sky, float* out_rgba32f,
+                            int32_t width, int32_t height, int32_t row_pitch_px, int32_t phase,
+                            const mm_march_params* params, unsigned long long* counts, void* stream) {
+    if (!counts) return MM_E_NULL;
+    return march_impl(camera, sun, sky, textures, night_sky, out_rgba32f, width, height, row_pitch_px, phase, params,
+                      counts, stream);
+}
+
+}  // extern "C"

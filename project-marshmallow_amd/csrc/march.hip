@@ -1,0 +1,635 @@
+// Cloud raymarch kernel for gfx950 (CDNA4), the HIP counterpart of the compute dispatch of
+// SkyEngine/SkyEngine/Shaders/compute-clouds.comp:288-500.
+//
+// One work-item per marched pixel, 64-lane wavefronts covering 8x8 pixel tiles (rays of a wave
+// are spatially coherent, so their step counts and texel footprints are close). Uniform-derived
+// values are hoisted to the host (MarchConsts). Texture filtering is done in software: LINEAR +
+// REPEAT with the texture unit's fixed-point sub-texel weights (8 bits) or exact float weights,
+// over one of three texel format sets:
+//   FSET_REF : every texture RGBA8 UNORM, linear layout (the reference's bindings, drop-in)
+//   FSET_FAST: weather RG8, curl RGBA8, low-res R8 + 16-bit erosion sum, hi-res 16-bit erosion sum,
+//              apron layout (N+1 texels per axis) so a filter footprint never wraps
+//   FSET_PREC: weather RG8, curl FP16x4, low-res FP8 e4m3 x4 with per-channel scale, hi-res FP16x4
+// Built with -ffp-contract=off and GLSL-grade division/sqrt (approximate reciprocal): every other
+// float operation rounds as the oracle's restatement does.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include "mm_internal.hpp"
+
+namespace {
+
+using mm::MarchConsts;
+using mm::MarchTextures;
+using mm::TexView;
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 mk(float x, float y, float z) {
+    V3 r;
+    r.x = x; r.y = y; r.z = z;
+    return r;
+}
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ V3 mul3(V3 a, V3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float dot3(V3 a, V3 b) { return ((a.x * b.x) + (a.y * b.y)) + (a.z * b.z); }
+__device__ __forceinline__ V3 ld3(const float* f) { return mk(f[0], f[1], f[2]); }
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ V3 normalize3(V3 a) { return a * rsq(dot3(a, a)); }
+__device__ __forceinline__ float length3(V3 a) { return sqrt_(dot3(a, a)); }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
+__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
+    float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+__device__ __forceinline__ float exp2_(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float log2_(float x) { return __builtin_amdgcn_logf(x); }
+// GLSL-grade pow/exp: exp2(y * log2(x)), exp2(x * log2(e)).
+__device__ __forceinline__ float powg(float x, float y) { return exp2_(y * log2_(x)); }
+__device__ __forceinline__ float expg(float x) { return exp2_(x * 1.4426950408889634f); }
+// remap with a constant range: (v - a) * (1 / (b - a)) * (d - c) + c
+__device__ __forceinline__ float remapk(float v, float a, float inv_ba, float c, float dc) {
+    return c + (((v - a) * inv_ba) * dc);
+}
+__device__ __forceinline__ float remapClamped01(float v, float a) {  // remapClamped(v, a, 1, 0, 1)
+    return clampf(0.0f + (((v - a) / (1.0f - a)) * 1.0f), 0.0f, 1.0f);
+}
+
+// ------------------------------------------------------------------ sampler
+// x = u * n - 0.5, optionally rounded to 1/256 (fixed-point sub-texel weights), i0 = floor(x) mod n
+// (n a power of two: two's-complement AND is the floor modulo), f = x - floor(x).
+__device__ __forceinline__ void axis_p2(float u, int n, int q8, int& i0, float& f) {
+    float x = u * (float)n - 0.5f;
+    if (q8) x = floorf(x * 256.0f + 0.5f) * (1.0f / 256.0f);
+    float fl = floorf(x);
+    f = x - fl;
+    i0 = ((int)fl) & (n - 1);
+}
+
+__device__ __forceinline__ uint32_t ld_u32(const uint8_t* p) { return *reinterpret_cast<const uint32_t*>(p); }
+__device__ __forceinline__ uint32_t ld_u16(const uint8_t* p) { return *reinterpret_cast<const uint16_t*>(p); }
+__device__ __forceinline__ uint2 ld_u64(const uint8_t* p) { return *reinterpret_cast<const uint2*>(p); }
+
+template <int BYTES> struct TexelT;
+template <> struct TexelT<2> { typedef uint32_t type; };
+template <> struct TexelT<4> { typedef uint32_t type; };
+template <> struct TexelT<8> { typedef uint2 type; };
+
+template <int BYTES>
+__device__ __forceinline__ typename TexelT<BYTES>::type ld_texel(const uint8_t* p) {
+    if constexpr (BYTES == 2) return ld_u16(p);
+    else if constexpr (BYTES == 4) return ld_u32(p);
+    else return ld_u64(p);
+}
+
+template <int BYTES> struct G2 {
+    typename TexelT<BYTES>::type t[4];
+    float fx, fy;
+};
+template <int BYTES> struct G3 {
+    typename TexelT<BYTES>::type t[8];
+    float fx, fy, fz;
+};
+
+// Gather of the 2x2 bilinear footprint. APRON: the texture stores n+1 texels per axis with texel n
+// equal to texel 0, so the +1 neighbour never wraps.
+template <int BYTES, bool APRON>
+__device__ __forceinline__ G2<BYTES> gather2(const TexView& tv, float u, float v, int q8) {
+    G2<BYTES> g;
+    int x0, y0;
+    axis_p2(u, tv.w, q8, x0, g.fx);
+    axis_p2(v, tv.h, q8, y0, g.fy);
+    int x1, y1;
+    if (APRON) { x1 = x0 + 1; y1 = y0 + 1; }
+    else { x1 = (x0 + 1) & (tv.w - 1); y1 = (y0 + 1) & (tv.h - 1); }
+    const uint8_t* r0 = tv.data + (size_t)(y0 * tv.stride) * BYTES;
+    const uint8_t* r1 = tv.data + (size_t)(y1 * tv.stride) * BYTES;
+    g.t[0] = ld_texel<BYTES>(r0 + x0 * BYTES);
+    g.t[1] = ld_texel<BYTES>(r0 + x1 * BYTES);
+    g.t[2] = ld_texel<BYTES>(r1 + x0 * BYTES);
+    g.t[3] = ld_texel<BYTES>(r1 + x1 * BYTES);
+    return g;
+}
+
+template <int BYTES, bool APRON>
+__device__ __forceinline__ G3<BYTES> gather3(const TexView& tv, float u, float v, float w, int q8) {
+    G3<BYTES> g;
+    int x0, y0, z0;
+    const int n = tv.w;
+    axis_p2(u, n, q8, x0, g.fx);
+    axis_p2(v, n, q8, y0, g.fy);
+    axis_p2(w, n, q8, z0, g.fz);
+    int x1, y1, z1;
+    if (APRON) { x1 = x0 + 1; y1 = y0 + 1; z1 = z0 + 1; }
+    else { x1 = (x0 + 1) & (n - 1); y1 = (y0 + 1) & (n - 1); z1 = (z0 + 1) & (n - 1); }
+    const uint8_t* b = tv.data;
+    const size_t r00 = (size_t)(z0 * tv.sliceStride + y0 * tv.stride);
+    const size_t r10 = (size_t)(z0 * tv.sliceStride + y1 * tv.stride);
+    const size_t r01 = (size_t)(z1 * tv.sliceStride + y0 * tv.stride);
+    const size_t r11 = (size_t)(z1 * tv.sliceStride + y1 * tv.stride);
+    g.t[0] = ld_texel<BYTES>(b + (r00 + x0) * BYTES);
+    g.t[1] = ld_texel<BYTES>(b + (r00 + x1) * BYTES);
+    g.t[2] = ld_texel<BYTES>(b + (r10 + x0) * BYTES);
+    g.t[3] = ld_texel<BYTES>(b + (r10 + x1) * BYTES);
+    g.t[4] = ld_texel<BYTES>(b + (r01 + x0) * BYTES);
+    g.t[5] = ld_texel<BYTES>(b + (r01 + x1) * BYTES);
+    g.t[6] = ld_texel<BYTES>(b + (r11 + x0) * BYTES);
+    g.t[7] = ld_texel<BYTES>(b + (r11 + x1) * BYTES);
+    return g;
+}
+
+template <int BYTES, class F>
+__device__ __forceinline__ float filt2(const G2<BYTES>& g, F ch) {
+    float a = lerpf(ch(g.t[0]), ch(g.t[1]), g.fx);
+    float b = lerpf(ch(g.t[2]), ch(g.t[3]), g.fx);
+    return lerpf(a, b, g.fy);
+}
+template <int BYTES, class F>
+__device__ __forceinline__ float filt3(const G3<BYTES>& g, F ch) {
+    float a0 = lerpf(ch(g.t[0]), ch(g.t[1]), g.fx);
+    float b0 = lerpf(ch(g.t[2]), ch(g.t[3]), g.fx);
+    float a1 = lerpf(ch(g.t[4]), ch(g.t[5]), g.fx);
+    float b1 = lerpf(ch(g.t[6]), ch(g.t[7]), g.fx);
+    float s0 = lerpf(a0, b0, g.fy);
+    float s1 = lerpf(a1, b1, g.fy);
+    return lerpf(s0, s1, g.fz);
+}
+
+// Channel decoders.
+template <int K> struct Byte {
+    __device__ __forceinline__ float operator()(uint32_t v) const { return (float)((v >> (8 * K)) & 0xffu); }
+};
+struct Hi16 {
+    __device__ __forceinline__ float operator()(uint32_t v) const { return (float)(v >> 16); }
+};
+struct Lo16 {
+    __device__ __forceinline__ float operator()(uint32_t v) const { return (float)(v & 0xffffu); }
+};
+template <int K> struct Half4 {  // channel K of 4 x FP16 packed in a uint2
+    __device__ __forceinline__ float operator()(uint2 v) const {
+        uint32_t w = (K < 2) ? v.x : v.y;
+        uint16_t h = (uint16_t)((K & 1) ? (w >> 16) : (w & 0xffffu));
+        return __half2float(__ushort_as_half(h));
+    }
+};
+template <int K> struct Fp8 {  // channel K of 4 x FP8 e4m3 (OCP) packed in a uint32
+    __device__ __forceinline__ float operator()(uint32_t v) const {
+        return __builtin_amdgcn_cvt_f32_fp8((int)v, K);
+    }
+};
+
+constexpr float kInv255 = 1.0f / 255.0f;
+constexpr float kInv2040 = 1.0f / 2040.0f;
+
+// ------------------------------------------------------------------ per-format texture reads
+struct Weather {
+    float coverage, type;
+};
+
+template <int FSET>
+__device__ __forceinline__ Weather read_weather(const TexView& tv, float u, float v, int q8) {
+    Weather w;
+    if constexpr (FSET == mm::FSET_REF) {
+        G2<4> g = gather2<4, false>(tv, u, v, q8);
+        w.coverage = filt2(g, Byte<0>()) * kInv255;
+        w.type = filt2(g, Byte<2>()) * kInv255;
+    } else {
+        G2<2> g = gather2<2, true>(tv, u, v, q8);
+        w.coverage = filt2(g, Byte<0>()) * kInv255;
+        w.type = filt2(g, Byte<1>()) * kInv255;
+    }
+    return w;
+}
+
+template <int FSET>
+__device__ __forceinline__ V3 read_curl(const TexView& tv, float u, float v, int q8) {
+    if constexpr (FSET == mm::FSET_PREC) {
+        G2<8> g = gather2<8, true>(tv, u, v, q8);
+        return mk(filt2(g, Half4<0>()), filt2(g, Half4<1>()), filt2(g, Half4<2>()));
+    } else {
+        G2<4> g = gather2<4, FSET != mm::FSET_REF>(tv, u, v, q8);
+        return mk(filt2(g, Byte<0>()) * kInv255, filt2(g, Byte<1>()) * kInv255, filt2(g, Byte<2>()) * kInv255);
+    }
+}
+
+// Hi-res erosion 0.625 r + 0.25 g + 0.125 b of the filtered texel (comp:223-224).
+template <int FSET>
+__device__ __forceinline__ float read_hi_erosion(const TexView& tv, V3 p, int q8) {
+    if constexpr (FSET == mm::FSET_REF) {
+        G3<4> g = gather3<4, false>(tv, p.x, p.y, p.z, q8);
+        float r = filt3(g, Byte<0>()) * kInv255, gg = filt3(g, Byte<1>()) * kInv255, b = filt3(g, Byte<2>()) * kInv255;
+        return ((0.625f * r) + (0.25f * gg)) + (0.125f * b);
+    } else if constexpr (FSET == mm::FSET_FAST) {
+        G3<2> g = gather3<2, true>(tv, p.x, p.y, p.z, q8);
+        return filt3(g, Lo16()) * kInv2040;
+    } else {
+        G3<8> g = gather3<8, true>(tv, p.x, p.y, p.z, q8);
+        float r = filt3(g, Half4<0>()), gg = filt3(g, Half4<1>()), b = filt3(g, Half4<2>());
+        return ((0.625f * r) + (0.25f * gg)) + (0.125f * b);
+    }
+}
+
+// Low-res read in two halves: the base shape first (so the < 1e-4 early out of comp:243 skips the
+// erosion decode), the erosion sum 0.625 g + 0.25 b + 0.125 a afterwards.
+template <int FSET> struct LowG;
+template <> struct LowG<mm::FSET_REF> { G3<4> g; };
+template <> struct LowG<mm::FSET_FAST> { G3<4> g; };
+template <> struct LowG<mm::FSET_PREC> { G3<4> g; };
+
+template <int FSET>
+__device__ __forceinline__ LowG<FSET> gather_low(const TexView& tv, V3 p, int q8) {
+    LowG<FSET> r;
+    r.g = gather3<4, FSET != mm::FSET_REF>(tv, p.x, p.y, p.z, q8);
+    return r;
+}
+template <int FSET>
+__device__ __forceinline__ float low_shape(const LowG<FSET>& l, const TexView& tv) {
+    if constexpr (FSET == mm::FSET_PREC) return filt3(l.g, Fp8<0>()) * tv.scale[0];
+    else return filt3(l.g, Byte<0>()) * kInv255;
+}
+template <int FSET>
+__device__ __forceinline__ float low_erosion(const LowG<FSET>& l, const TexView& tv) {
+    if constexpr (FSET == mm::FSET_REF) {
+        float g = filt3(l.g, Byte<1>()) * kInv255, b = filt3(l.g, Byte<2>()) * kInv255, a = filt3(l.g, Byte<3>()) * kInv255;
+        return ((0.625f * g) + (0.25f * b)) + (0.125f * a);
+    } else if constexpr (FSET == mm::FSET_FAST) {
+        return filt3(l.g, Hi16()) * kInv2040;
+    } else {
+        float g = filt3(l.g, Fp8<1>()) * tv.scale[1], b = filt3(l.g, Fp8<2>()) * tv.scale[2],
+              a = filt3(l.g, Fp8<3>()) * tv.scale[3];
+        return ((0.625f * g) + (0.25f * b)) + (0.125f * a);
+    }
+}
+
+// ------------------------------------------------------------------ shader functions
+// getProjectedShellPoint (comp:180-182)
+__device__ __forceinline__ V3 proj_shell(const MarchConsts& c, V3 pt, V3 center) {
+    return (c.projRadius * normalize3(pt - center)) + center;
+}
+// getRelativeHeight (comp:186-188)
+__device__ __forceinline__ float rel_height(const MarchConsts& c, V3 pt, V3 proj) {
+    return clampf(length3(pt - proj) * c.invThickness, 0.0f, 1.0f);
+}
+
+// cloudLayerDensity (comp:193-204)
+__device__ __forceinline__ float layer_density(float h, float type) {
+    h = clampf(h, 0.0f, 1.0f);
+    const float r02 = remapk(h, 0.0f, 1.0f / 0.2f, 0.0f, 1.0f);
+    float cumulus = fmaxf(0.0f, r02 * remapk(h, 0.7f, 1.0f / (0.9f - 0.7f), 1.0f, -1.0f));
+    float strato = fmaxf(0.0f, r02 * remapk(h, 0.2f, 1.0f / (0.7f - 0.2f), 1.0f, -1.0f));
+    float stratus = fmaxf(0.0f, remapk(h, 0.0f, 1.0f / 0.1f, 0.0f, 1.0f) * remapk(h, 0.2f, 1.0f / (0.3f - 0.2f), 1.0f, -1.0f));
+    float d1 = mixf(stratus, strato, clampf(type * 2.0f, 0.0f, 1.0f));
+    float d2 = mixf(strato, cumulus, clampf((type - 0.5f) * 2.0f, 0.0f, 1.0f));
+    return mixf(d1, d2, type);
+}
+
+// wind offset (comp:414, 445)
+__device__ __forceinline__ V3 wind_offset(const MarchConsts& c, float h) {
+    V3 w = mk(c.wind[0] + h * 0.1f, c.wind[1] + h * 0.05f, c.wind[2] + h * 0.0f);
+    return (c.windStrength * w) * (c.time + h * 200.0f);
+}
+
+template <int FSET, bool COUNT>
+__device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTextures& tx, V3 pos, float h, V3 center,
+                                            int& nd) {
+    if (COUNT) nd++;
+    const int q8 = c.subtexelBits;
+    V3 proj = proj_shell(c, pos, center);
+    LowG<FSET> lg = gather_low<FSET>(tx.low, pos * c.lowScale, q8);
+    Weather w = read_weather<FSET>(tx.weather, c.weatherScale * (proj.x - c.camPos[0]),
+                                   c.weatherScale * (proj.z - c.camPos[2]), q8);
+    float layer = layer_density(h, w.type);
+    float shape = low_shape<FSET>(lg, tx.low);
+    float density = layer * clampf(0.0f + (((shape - 0.3f) * (1.0f / (1.0f - 0.3f))) * 1.0f), 0.0f, 1.0f);
+    if (density < 0.0001f) return 0.0f;
+    // heightBiasCoverage(relativeHeight, min(0.85, cloudInfo.r)) (comp:206-208, 245)
+    float e = clampf(remapk(fminf(0.85f, w.coverage), 0.7f, 1.0f / (0.8f - 0.7f), 1.0f, 0.8f - 1.0f), 0.8f, 1.0f);
+    float coverage = powg(h, e);
+    float erosion = low_erosion<FSET>(lg, tx.low);
+    erosion = remapClamped01(erosion, coverage);
+    return remapClamped01(density, erosion);
+}
+
+template <int FSET, bool COUNT>
+__device__ __forceinline__ float cloud_hires(const MarchConsts& c, const MarchTextures& tx, V3 pos, float curlStrength,
+                                             float orig, float h, int& nh) {
+    if (COUNT) nh++;
+    const int q8 = c.subtexelBits;
+    V3 cv = read_curl<FSET>(tx.curl, c.curlScale * pos.x, c.curlScale * pos.z, q8);
+    V3 curl = mk(2.0f * cv.x - 1.0f, 2.0f * cv.y - 1.0f, 2.0f * cv.z - 1.0f);
+    pos = pos + ((1.9f * curlStrength) * curl);
+    float e = read_hi_erosion<FSET>(tx.hi, pos * c.hiScale, q8);
+    e = mixf(e, 1.0f - e, clampf(h * 10.0f, 0.0f, 1.0f));
+    return remapClamped01(orig, 1.0f * e);
+}
+
+// hgPhase (comp:73-77)
+__device__ __forceinline__ float hg_phase(float cosTheta, float g) {
+    float g2 = g * g;
+    float inv = 1.0f / powg(((1.0f - (2.0f * g) * cosTheta) + g2), 1.5f);
+    return 0.07957747154594767f * ((1.0f - g2) * inv);
+}
+
+// getAtmosphereColorPhysical (comp:88-127)
+__device__ V3 atmosphere(const MarchConsts& c, V3 dir, V3 sunDir) {
+    const float PI = 3.14159265f;
+    const float sunE = c.sunIntensity;
+    float zenith = acosf(fmaxf(0.0f, dir.y));
+    float inverse = 1.0f / (cosf(zenith) + 0.15f * powg(93.885f - ((zenith * 180.0f) / PI), -1.253f));
+    float sR = 8.4e3f * inverse;
+    float sM = 1.25e3f * inverse;
+    float fex[3], Lin[3];
+    float cosTheta = dot3(sunDir, dir);
+    float rc = cosTheta * 0.5f + 0.5f;
+    float rPhase = 0.05968310365946075f * (1.0f + rc * rc);
+    float mPhase = hg_phase(cosTheta, c.mieG);
+    float yDot = 1.0f - sunDir.y;
+    yDot = yDot * (((yDot * yDot) * yDot) * yDot);
+    float yc = clampf(yDot, 0.0f, 1.0f);
+    V3 out;
+    float* o = &out.x;
+    for (int k = 0; k < 3; k++) {
+        fex[k] = expg((c.betaR[k] * (-sR)) + (c.betaM[k] * sM));
+        float betas = ((c.betaR[k] * rPhase) + (c.betaM[k] * mPhase)) / (c.betaR[k] + c.betaM[k]);
+        float a = (betas * sunE) * (1.0f - fex[k]);
+        Lin[k] = powg(a, 1.5f);
+        float b = (betas * sunE) * fex[k];
+        Lin[k] = Lin[k] * mixf(1.0f, powg(b, 0.5f), yc);
+        float L0 = fex[k] * 0.1f;
+        o[k] = (Lin[k] + L0) * 0.04f;
+    }
+    out.y = out.y + 0.0003f;
+    out.z = out.z + 0.00075f;
+    out.x = out.x + 0.0f;
+    return out;
+}
+
+// raySphereIntersection (comp:147-177) in arithmetic type I; a miss yields t = 0.
+template <class I>
+__device__ __forceinline__ I sphere_t(I rox, I roy, I roz, I rdx, I rdy, I rdz, I cx, I cy, I cz, I w, bool& valid) {
+    rox = (rox - cx) / w; roy = (roy - cy) / w; roz = (roz - cz) / w;
+    I A = (rdx * rdx + rdy * rdy) + rdz * rdz;
+    I B = I(2) * ((rdx * rox + rdy * roy) + rdz * roz);
+    I C = ((rox * rox + roy * roy) + roz * roz) - I(0.25);
+    I disc = (B * B) - ((I(4) * A) * C);
+    valid = false;
+    if (disc < I(0)) return I(0);
+    I sq = sqrt(disc);
+    I t = ((-sq - B) / A) * I(0.5);
+    if (t < I(0)) t = ((sq - B) / A) * I(0.5);
+    if (!(t >= I(0))) return I(0);
+    valid = true;
+    I px = (rox + rdx * t) * w + cx, py = (roy + rdy * t) * w + cy, pz = (roz + rdz * t) * w + cz;
+    I dx = px - rox, dy = py - roy, dz = pz - roz;
+    return sqrt((dx * dx + dy * dy) + dz * dz);
+}
+
+__device__ __forceinline__ float pow_chain4(float d) { return d * ((d * d) * d); }
+
+// Bilinear RGBA8 night-sky fetch with generic (non power of two) REPEAT wrap.
+__device__ V3 night_sky(const MarchConsts& c, const MarchTextures& tx, float u, float v) {
+    if (!tx.hasNight) return mk(0.f, 0.f, 0.f);
+    const TexView& tv = tx.night;
+    const int q8 = c.subtexelBits;
+    float f[2];
+    int i0[2], i1[2];
+    const float uv[2] = {u, v};
+    const int n[2] = {tv.w, tv.h};
+    for (int a = 0; a < 2; a++) {
+        float x = uv[a] * (float)n[a] - 0.5f;
+        if (q8) x = floorf(x * 256.0f + 0.5f) * (1.0f / 256.0f);
+        float fl = floorf(x);
+        f[a] = x - fl;
+        int ii = (int)fl % n[a];
+        if (ii < 0) ii += n[a];
+        i0[a] = ii;
+        i1[a] = (ii + 1 == n[a]) ? 0 : ii + 1;
+    }
+    const uint8_t* r0 = tv.data + (size_t)i0[1] * tv.stride * 4;
+    const uint8_t* r1 = tv.data + (size_t)i1[1] * tv.stride * 4;
+    G2<4> g;
+    g.t[0] = ld_u32(r0 + i0[0] * 4); g.t[1] = ld_u32(r0 + i1[0] * 4);
+    g.t[2] = ld_u32(r1 + i0[0] * 4); g.t[3] = ld_u32(r1 + i1[0] * 4);
+    g.fx = f[0]; g.fy = f[1];
+    return mk(filt2(g, Byte<0>()) * kInv255, filt2(g, Byte<1>()) * kInv255, filt2(g, Byte<2>()) * kInv255);
+}
+
+// ------------------------------------------------------------------ kernel
+template <int FSET, bool COUNT>
+__global__ __launch_bounds__(256) void march_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
+                                                    unsigned long long* __restrict__ counts) {
+    // block = 16x16 marched pixels as 4 waves of 8x8
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tileX = blockIdx.x % c.tilesX, tileY = blockIdx.x / c.tilesX;
+    const int gi = tileX * 16 + (wave & 1) * 8 + (lane & 7);
+    const int gj = tileY * 16 + (wave >> 1) * 8 + (lane >> 3);
+    if (gi >= c.gridW || gj >= c.gridH) return;
+    int px, py;
+    if (c.phase >= 0) {
+        px = gi * 4 + (c.phase & 3);
+        py = gj * 4 + (c.phase >> 2);
+    } else {
+        px = gi;
+        py = gj;
+    }
+    int nd = 0, nh = 0;
+
+    // ray generation (comp:303-322)
+    float uvx = (float)px / (float)c.width, uvy = (float)py / (float)c.height;
+    float sx = uvx * 2.0f - 1.0f, sy = uvy * 2.0f - 1.0f;
+    const V3 camPos = ld3(c.camPos);
+    V3 p = (ld3(c.refPoint) + (((c.aspect * sx) * c.tanH) * ld3(c.camRight))) - ((sy * c.tanH) * ld3(c.camUp));
+    const V3 rd = normalize3(p - camPos);
+    const V3 sunDir = ld3(c.sunDir);
+
+    // sun glow / alpha (comp:324-340)
+    const float SUN_COS = 0.999956676946448443553574619906976478926848692873900859324f;
+    float dotToSun = fmaxf(0.0f, dot3(sunDir, rd));
+    float skyAmbient = dotToSun * 0.18f;
+    skyAmbient = skyAmbient * (skyAmbient * skyAmbient);
+    float sunDisk = smoothstepf(SUN_COS, SUN_COS + 0.00003f, dotToSun);
+    dotToSun = pow_chain4(pow_chain4(pow_chain4(pow_chain4(dotToSun))));
+    dotToSun = dotToSun * (dotToSun * dotToSun);
+    const bool night = c.sunDirY < 0.0f;
+    if (night) {
+        float d = dotToSun;
+        dotToSun = d * ((((((d * d) * d) * d) * d) * d) * d);
+    }
+    sunDisk = fmaxf(0.0f, fmaxf(sunDisk, dotToSun));
+
+    float4 fc = make_float4(0.f, 0.f, 0.f, 0.f);
+    V3 bg = mk(0.f, 0.f, 0.f);
+    if (!night) {
+        bg = atmosphere(c, rd, sunDir);
+        fc = make_float4(bg.x, bg.y, bg.z, fmaxf(skyAmbient, sunDisk));
+    }
+    float4* dst = reinterpret_cast<float4*>(out) + ((size_t)py * c.pitch + px);
+    if (rd.y < 0.0f) {  // comp:351-354
+        *dst = fc;
+        return;
+    }
+
+    V3 center = camPos;
+    center.y = c.earthCenter[1];
+    float tStart, tEnd;
+    {
+        bool vi, vo;
+        if (c.flags & MM_MARCH_ISECT_FP64) {
+            double ti = sphere_t<double>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
+                                         (double)c.innerDiam, vi);
+            double to = sphere_t<double>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
+                                         (double)c.outerDiam, vo);
+            tStart = (float)ti;
+            tEnd = (float)to;
+        } else {
+            tStart = sphere_t<float>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
+                                     c.innerDiam, vi);
+            tEnd = sphere_t<float>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
+                                   c.outerDiam, vo);
+        }
+    }
+
+    if (night) {  // comp:365-384
+        V3 c0 = ld3(c.nightRot[0]), c1 = ld3(c.nightRot[1]), c2 = ld3(c.nightRot[2]);
+        V3 rrd = ((c0 * rd.x) + (c1 * rd.y)) + (c2 * rd.z);
+        V3 rro = ((c0 * camPos.x) + (c1 * camPos.y)) + (c2 * camPos.z);
+        V3 point = (rrd * tEnd) + rro;
+        V3 pr = proj_shell(c, point, center);
+        float u = 0.00002f * (pr.x - camPos.x) + 0.35f;
+        float v = 0.00002f * (pr.z - camPos.z) + 0.35f;
+        bg = night_sky(c, tx, u, v);
+        bg = mk(bg.x * (sqrt_(bg.x) * 0.75f), bg.y * (sqrt_(bg.y) * 0.75f), bg.z * (sqrt_(bg.z) * 0.75f));
+        bg = mk(powg(bg.x, 2.2f), powg(bg.y, 2.2f), powg(bg.z, 2.2f));
+        bg = bg * 10.0f;
+        float falloff = powg(rd.y, 6.0f);
+        bg = bg * falloff;
+        float pw = powg(rd.y, 0.03125f);
+        bg = mk(mixf(0.3f * 0.05f, bg.x, pw), mixf(0.6f * 0.05f, bg.y, pw), mixf(4.0f * 0.05f, bg.z, pw));
+        bg = bg + mk(sunDisk, sunDisk, sunDisk);
+        fc.w = sunDisk;
+    }
+
+    // march (comp:386-482)
+    const float cosTheta = dot3(rd, sunDir);
+    float accum = 0.0f, transmittance = 1.0f;
+    float stepSize = c.stepSize0;
+    const float HG = fmaxf(hg_phase(cosTheta, 0.6f), 0.7f * hg_phase(cosTheta, 0.99f - 0.1f));
+    bool noHits = true;
+    int misses = 0, steps = 0, modeCount0 = 0, modeCount1 = 0;
+    const bool bounded = (c.flags & MM_MARCH_BOUNDED) != 0;
+
+    for (float t = tStart; t < tEnd; t += stepSize) {
+        const bool coarse = noHits;
+        if (bounded) {
+            if (coarse) { if (c.maxCoarse >= 0 && ++modeCount0 > c.maxCoarse) break; }
+            else { if (c.maxFine >= 0 && ++modeCount1 > c.maxFine) break; }
+        }
+        V3 currentPos = camPos + (t * rd);
+        V3 currentProj = proj_shell(c, currentPos, center);
+        float rHeight = rel_height(c, currentPos, currentProj);
+        V3 wo = wind_offset(c, rHeight);
+        float density = cloud_test<FSET, COUNT>(c, tx, currentPos + wo, rHeight, center, nd);
+        float loDensity = density;
+        if (density > 0.0f) {
+            misses = 0;
+            if (noHits) {
+                t -= stepSize;
+                stepSize *= c.fineMul;
+                noHits = false;
+                continue;
+            }
+            density = cloud_hires<FSET, COUNT>(c, tx, currentPos + wo, stepSize, density, rHeight, nh);
+            if (density < 0.0001f) continue;
+            float dal = 0.0f;
+#pragma unroll 1
+            for (int i = 0; i < 6; i++) {
+                V3 lsPos = currentPos + ((3.0f * stepSize) * ld3(c.coneOffsets[i]));
+                V3 lsProj = proj_shell(c, lsPos, center);
+                float lsHeight = rel_height(c, lsPos, lsProj);
+                V3 lwo = wind_offset(c, lsHeight);
+                float ls = cloud_test<FSET, COUNT>(c, tx, lsPos + lwo, lsHeight, center, nd);
+                if (ls > 0.0f) dal += cloud_hires<FSET, COUNT>(c, tx, lsPos + lwo, stepSize, ls, lsHeight, nh);
+            }
+            float beers = expg(-dal);
+            float beersMod = fmaxf(beers, 0.7f * expg(-0.25f * dal));
+            beers = mixf(beers, beersMod, (-cosTheta) * 0.5f + 0.5f);
+            float e1 = clampf(remapk(rHeight, 0.3f, 1.0f / (0.85f - 0.3f), 0.5f, 2.0f - 0.5f), 0.5f, 2.0f);
+            float inScatter = 0.09f + powg(loDensity, e1);
+            float e2 = clampf(remapk(rHeight, 0.07f, 1.0f / (0.34f - 0.07f), 0.1f, 1.0f - 0.1f), 0.1f, 1.0f);
+            inScatter *= powg(e2, 0.8f);
+            transmittance = mixf(transmittance, (inScatter * HG) * beers, 1.0f - accum);
+            accum += density;
+        } else if (!noHits) {
+            misses++;
+            if (misses >= c.missLimit) {
+                noHits = true;
+                stepSize /= c.fineMul;
+            }
+        }
+        if (accum > c.opacityCutoff) {
+            accum = 1.0f;
+            break;
+        }
+        if (c.maxSteps >= 0 && ++steps > c.maxSteps) break;
+        if (!bounded) {
+            if (coarse) { if (c.maxCoarse >= 0 && ++modeCount0 > c.maxCoarse) break; }
+            else { if (c.maxFine >= 0 && ++modeCount1 > c.maxFine) break; }
+        }
+    }
+
+    // compose (comp:484-499)
+    accum *= smoothstepf(0.0f, 1.0f, fminf(1.0f, remapk(rd.y, 0.0f, 1.0f / 0.1f, 0.0f, 1.0f)));
+    accum = fminf(accum, 0.999f);
+    const float I = c.sunIntensity;
+    const float e = expg(-transmittance);
+    V3 amb;
+    if (!night) amb = (bg * 0.08f) * e;
+    else {
+        float pw = powg(rd.y, 0.03125f);
+        amb = (mk(0.3f * 0.05f, 0.6f * 0.05f, 4.0f * 0.05f) * pw * 0.08f) * e;
+    }
+    const float direct = I * fmaxf(0.0f, transmittance);
+    V3 cloud = mk(c.sunColor[0] * (direct + amb.x), c.sunColor[1] * (direct + amb.y), c.sunColor[2] * (direct + amb.z));
+    float4 res;
+    res.x = mixf(bg.x, cloud.x, accum);
+    res.y = mixf(bg.y, cloud.y, accum);
+    res.z = mixf(bg.z, cloud.z, accum);
+    res.w = fc.w * fmaxf(1.0f - accum, 0.0f);
+    *dst = res;
+    if (COUNT) {
+        atomicAdd(&counts[0], (unsigned long long)nd);
+        atomicAdd(&counts[1], (unsigned long long)nh);
+    }
+}
+
+template <int FSET, bool COUNT>
+int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts, hipStream_t s) {
+    const int tilesY = (c.gridH + 15) / 16;
+    dim3 grid((unsigned)(c.tilesX * tilesY)), block(256);
+    hipLaunchKernelGGL((march_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
+    return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
+}
+
+}  // namespace
+
+namespace mm {
+
+int launch_march(int fset, const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts,
+                 void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const bool count = counts != nullptr;
+    switch (fset) {
+        case FSET_REF: return count ? launch_t<FSET_REF, true>(c, t, out, counts, s) : launch_t<FSET_REF, false>(c, t, out, counts, s);
+        case FSET_FAST: return count ? launch_t<FSET_FAST, true>(c, t, out, counts, s) : launch_t<FSET_FAST, false>(c, t, out, counts, s);
+        case FSET_PREC: return count ? launch_t<FSET_PREC, true>(c, t, out, counts, s) : launch_t<FSET_PREC, false>(c, t, out, counts, s);
+    }
+    return MM_E_FORMAT;
+}
+
+}  // namespace mm

@@ -1,0 +1,66 @@
+// Internal declarations shared by the C-ABI layer and the HIP launchers.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+#include "../../include/mm.h"
+
+namespace mm {
+
+// Per-dispatch constants derived once on the host from the uniform blocks, exactly as the shader
+// derives them per invocation (compute-clouds.comp:289-401). Passed by value as a kernel argument.
+struct MarchConsts {
+    float camPos[3];
+    float camLook[3], camRight[3], camUp[3];
+    float refPoint[3];           // cameraPos - camLook (comp:318)
+    float aspect, tanH;          // cameraParams.xy
+    float sunDir[3];             // normalize(directionBasis[1].xyz) (comp:324)
+    float coneOffsets[6][3];     // basis * cone table (comp:392-401)
+    float earthCenter[3];        // (cam.x, center_y, cam.z) (comp:357-358)
+    float innerDiam, outerDiam, thickness, invThickness, projRadius;
+    float stepSize0;             // coarse_step_frac * thickness (comp:390)
+    float fineMul;
+    float betaR[3], betaM[3];
+    float sunIntensity;
+    float sunColor[3];
+    float sunDirY;               // sun.direction.y (day/night switch, comp:336, 344, 365)
+    float mieG;
+    float wind[3], time, windStrength;
+    float weatherScale, lowScale, curlScale, hiScale;
+    float opacityCutoff;
+    float nightRot[3][3];        // fromAngleAxis(normalize(1,0,1), sun.direction.y * 0.5) columns
+    int missLimit, maxSteps, maxCoarse, maxFine;
+    int flags;
+    int subtexelBits;
+    int width, height, pitch;    // image size used for uv and bounds, row pitch in pixels
+    int phase;                   // -1 = all pixels
+    int gridW, gridH;            // marched pixel grid
+    int tilesX;
+};
+
+// Device view of one texture.
+struct TexView {
+    const uint8_t* data;
+    int w, h, d;                 // logical dims
+    int stride;                  // texels per row including the apron (w or w + 1)
+    int sliceStride;             // texels per slice including the apron
+    float scale[4];
+};
+
+enum FormatSet { FSET_REF = 0, FSET_FAST = 1, FSET_PREC = 2 };
+
+struct MarchTextures {
+    TexView weather, curl, low, hi, night;
+    int hasNight;
+};
+
+int launch_march(int fset, const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts,
+                 void* stream);
+
+// Noise generation launchers.
+int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream);
+size_t noise_workspace_bytes();
+int launch_pack(int kind, const mm_texture* src, const mm_texture* dst, void* stream);
+
+bool texture_dims_ok(const mm_texture* t, int dims);
+
+}  // namespace mm

@@ -1,0 +1,230 @@
+"""GPU cloud march (mm_march_clouds) against the CPU restatement of compute-clouds.comp.
+
+Bar (north star): after the reference's tonemap, >= 99.9 % of pixels within 1/255 per channel and
+no pixel off by more than 4/255 (alpha compared raw). Inputs: oracle-generated RGBA8 textures
+(seeded), uploaded as-is for the reference format set and repacked on the GPU for the packed sets.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import parity_stats
+
+pytestmark = pytest.mark.gpu
+
+TOL_FRAC, TOL_MAX = 0.999, 4.0
+
+
+@pytest.fixture(scope="module")
+def tex(orc):
+    return {"weather": orc.weather(512, 1, 0.5), "curl": orc.curl(128)[0], "low": orc.low_res(128, 1),
+            "hi": orc.hi_res(32, 1)}
+
+
+def upload(mm, tex, fset):
+    """Noise set in format set `fset` holding exactly the oracle's textures."""
+    import torch
+
+    ref = mm.NoiseSet("ref")
+    for slot, key in (("weather", "weather"), ("curl", "curl"), ("low_res", "low"), ("hi_res", "hi")):
+        getattr(ref, slot).tensor.copy_(torch.from_numpy(tex[key].reshape(-1)))
+    if fset == "ref":
+        return ref
+    ns = mm.NoiseSet(fset)
+    for kind, slot in enumerate(("weather", "curl", "low_res", "hi_res")):
+        mm.check(mm.lib().mm_pack_texture(kind, ctypes.byref(getattr(ref, slot).desc),
+                                          ctypes.byref(getattr(ns, slot).desc), mm.current_stream()), "pack")
+    torch.cuda.synchronize()
+    return ns
+
+
+def gpu_march(mm, ns, cam, sun, sky, W, H, phase=-1, params=None, night=None, fill=0.0, counts=False):
+    import torch
+
+    out = torch.full((H * W * 4,), fill, dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(2, dtype=torch.int64, device="cuda") if counts else None
+    mm.march_clouds(cam, sun, sky, ns, out, W, H, phase=phase, params=params, night=night, counts=cnt)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy().reshape(H, W, 4)
+    return (img, cnt.cpu().numpy()) if counts else img
+
+
+def check_parity(g, r, what):
+    st = parity_stats(g, r)
+    print(what, st, flush=True)
+    assert st["nan"] == 0
+    assert st["frac_within_1"] >= TOL_FRAC, (what, st)
+    assert st["max"] <= TOL_MAX, (what, st)
+    return st
+
+
+@pytest.mark.parametrize("fset", ["ref", "fast"])
+@pytest.mark.parametrize("preset", ["reference", "scenario"])
+def test_parity_256(mm, gpu, orc, tex, fset, preset):
+    W = H = 256
+    scene = mm.scenario_scene(aspect=1.0)
+    cam, sun, sky = mm.build_ubos(scene)
+    prm = mm.march_params(preset)
+    g = gpu_march(mm, upload(mm, tex, fset), cam, sun, sky, W, H, params=prm)
+    r = orc.march(cam, sun, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]), prm, W, H)
+    check_parity(g, r, f"{preset}/{fset}")
+
+
+@pytest.mark.parametrize("case", ["fp32_isect", "float_weights", "camera_in_shell", "low_sun", "wind_t600"])
+def test_parity_variants(mm, gpu, orc, tex, case):
+    W = H = 192
+    prm = mm.march_params("scenario")
+    scene = mm.scenario_scene(aspect=1.0)
+    if case == "fp32_isect":
+        prm = mm.march_params("reference")
+    elif case == "float_weights":
+        prm.subtexel_bits = 0
+    elif case == "camera_in_shell":
+        scene = mm.scenario_scene(altitude=2500.0, aspect=1.0)  # inside the 1500-4000 m shell: march from t = 0
+    elif case == "low_sun":
+        scene = mm.scenario_scene(sun_elevation_deg=5.0, aspect=1.0)
+    elif case == "wind_t600":
+        scene = mm.scenario_scene(time=600.0, aspect=1.0)
+    cam, sun, sky = mm.build_ubos(scene)
+    g = gpu_march(mm, upload(mm, tex, "fast"), cam, sun, sky, W, H, params=prm)
+    r = orc.march(cam, sun, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]), prm, W, H)
+    check_parity(g, r, case)
+
+
+def test_night_branch(mm, gpu, orc, tex):
+    import torch
+
+    W = H = 128
+    rng = np.random.default_rng(3)
+    night = rng.integers(0, 256, (90, 160, 4), dtype=np.uint8)
+    nt = mm.device_texture(160, 90, 1, mm.FMT_RGBA8, mm.LAYOUT_LINEAR)
+    nt.tensor.copy_(torch.from_numpy(night.reshape(-1)))
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(sun_elevation_deg=-12.0, aspect=1.0))
+    prm = mm.march_params("reference")
+    g = gpu_march(mm, upload(mm, tex, "ref"), cam, sun, sky, W, H, params=prm, night=nt)
+    r = orc.march(cam, sun, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"], night), prm, W, H)
+    check_parity(g, r, "night")
+
+
+def test_phase_writes_only_its_pixels_and_composes(mm, gpu, tex):
+    W, H = 70, 45  # ragged: not multiples of 4
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H))
+    prm = mm.march_params("scenario")
+    ns = upload(mm, tex, "fast")
+    full = gpu_march(mm, ns, cam, sun, sky, W, H, phase=-1, params=prm)
+    acc = np.full((H, W, 4), np.nan, np.float32)
+    for p in range(16):
+        img = gpu_march(mm, ns, cam, sun, sky, W, H, phase=p, params=prm, fill=np.nan)
+        written = ~np.isnan(img[..., 0])
+        yy, xx = np.nonzero(written)
+        assert ((xx % 4) == p % 4).all() and ((yy % 4) == p // 4).all()
+        assert written.sum() == len(range(p % 4, W, 4)) * len(range(p // 4, H, 4))
+        acc[written] = img[written]
+    assert np.array_equal(acc, full)  # 16 phase dispatches == one full frame, bit for bit
+
+
+def test_phase_from_ubo(mm, gpu, tex):
+    W = H = 64
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=1.0, phase=9))
+    ns = upload(mm, tex, "fast")
+    a = gpu_march(mm, ns, cam, sun, sky, W, H, phase=mm.PHASE_FROM_UBO, fill=np.nan)
+    b = gpu_march(mm, ns, cam, sun, sky, W, H, phase=9, fill=np.nan)
+    assert np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(np.nan_to_num(a), np.nan_to_num(b))
+
+
+def test_sample_counts_match_oracle(mm, gpu, orc, tex):
+    W = H = 128
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=1.0))
+    prm = mm.march_params("scenario")
+    _, cnt = gpu_march(mm, upload(mm, tex, "fast"), cam, sun, sky, W, H, params=prm, counts=True)
+    _, rc = orc.march(cam, sun, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]), prm, W, H,
+                      counts=True)
+    d_ref, h_ref = int(rc[..., 0].sum()), int(rc[..., 1].sum())
+    assert d_ref > 0 and h_ref > 0
+    assert abs(int(cnt[0]) - d_ref) <= 0.002 * d_ref
+    assert abs(int(cnt[1]) - h_ref) <= 0.002 * h_ref
+
+
+def test_ray_invariants(mm, gpu, orc, tex):
+    import torch
+
+    W = H = 96
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=1.0))
+    prm = mm.march_params("scenario")
+    # zero base shape: no density anywhere, so RGB is the sky and alpha the sun glow
+    empty = dict(tex, low=np.zeros_like(tex["low"]))
+    g = gpu_march(mm, upload(mm, empty, "ref"), cam, sun, sky, W, H, params=prm)
+    r = orc.march(cam, sun, sky, orc.textures(empty["weather"], empty["curl"], empty["low"], empty["hi"]), prm, W, H)
+    check_parity(g, r, "empty")
+    # full base shape and no erosion: density = layer profile, so every ray crossing the shell well
+    # above the horizon saturates (accum clamped to 0.999, comp:476-486) and alpha drops to 1e-3 glow
+    low = np.zeros_like(tex["low"])
+    low[..., 0] = 255
+    full = dict(tex, low=low, hi=np.full_like(tex["hi"], 255))
+    g2 = gpu_march(mm, upload(mm, full, "ref"), cam, sun, sky, W, H, params=prm)
+    r2 = orc.march(cam, sun, sky, orc.textures(full["weather"], full["curl"], full["low"], full["hi"]), prm, W, H)
+    check_parity(g2, r2, "saturated")
+    top = slice(0, H // 3)
+    assert (g2[top, :, 3] <= 0.0011 * g[top, :, 3] + 1e-7).all()
+
+
+def test_1080p_properties_and_sampled_parity(mm, gpu, orc, tex):
+    W, H = 1920, 1080
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H))
+    prm = mm.march_params("scenario")
+    ns = upload(mm, tex, "fast")
+    a = gpu_march(mm, ns, cam, sun, sky, W, H, params=prm)
+    b = gpu_march(mm, ns, cam, sun, sky, W, H, params=prm)
+    assert np.array_equal(a, b)  # deterministic
+    assert np.isfinite(a).all() and (a[..., 3] >= 0).all() and (a[..., 3] <= 1).all()
+    rng = np.random.default_rng(1)
+    xy = np.stack([rng.integers(0, W, 4096), rng.integers(0, H, 4096)], 1).astype(np.int32)
+    r = orc.march_pixels(cam, sun, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]), prm, W, H,
+                         xy)
+    check_parity(a[xy[:, 1], xy[:, 0]], r, "1080p-sampled")
+
+
+def test_errors_with_real_buffers(mm, gpu, tex):
+    import torch
+
+    ns = upload(mm, tex, "fast")
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene())
+    out = torch.zeros(64 * 64 * 4, dtype=torch.float32, device="cuda")
+    with pytest.raises(mm.MMError) as e:
+        mm.march_clouds(cam, sun, sky, ns, out, 64, 64, pitch=32)
+    assert e.value.code == mm.MM_E_PITCH
+    with pytest.raises(mm.MMError) as e:
+        mm.march_clouds(cam, sun, sky, ns, out, 64, 64, phase=20)
+    assert e.value.code == mm.MM_E_PHASE
+    sun.directionBasis[5] = float("inf")
+    with pytest.raises(mm.MMError) as e:
+        mm.march_clouds(cam, sun, sky, ns, out, 64, 64)
+    assert e.value.code == mm.MM_E_PARAM
+    # a mixed format set is refused
+    ref = mm.NoiseSet("ref")
+    mixed = mm.NoiseSetC(ns.weather.desc, ref.curl.desc, ns.low_res.desc, ns.hi_res.desc)
+    c2, s2, k2 = mm.build_ubos(mm.scenario_scene())
+    code = mm.lib().mm_march_clouds(ctypes.byref(c2), ctypes.byref(s2), ctypes.byref(k2), ctypes.byref(mixed), None,
+                                    out.data_ptr(), 64, 64, 64, -1, None, mm.current_stream())
+    assert code == mm.MM_E_FORMAT
+
+
+def test_compute_shader_operator_ping_pong(mm, gpu, tex):
+    W = H = 64
+    ns = upload(mm, tex, "fast")
+    op = mm.ComputeShader(W, H, ns, params=mm.march_params("scenario"))
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=1.0, phase=0))
+    op.update_uniform_buffers(cam, cam, sky, sun)
+    a = op.dispatch()
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=1.0, phase=5))
+    op.update_uniform_buffers(cam, cam, sky, sun)
+    b = op.dispatch()
+    assert a.data_ptr() != b.data_ptr()  # storage images alternate like storageBufferSetA/B
+    import torch
+
+    torch.cuda.synchronize()
+    ai = a.cpu().numpy().reshape(H, W, 4)
+    bi = b.cpu().numpy().reshape(H, W, 4)
+    assert (ai[0::4, 0::4, 3] >= 0).all() and ai[1::4, 1::4].sum() == 0  # phase 0 only
+    assert bi[1::4, 1::4, 3].max() > 0 and bi[0::4, 0::4].sum() == 0      # phase 5 only
