@@ -196,6 +196,11 @@ void mm_march_params_reference(mm_march_params* p);
  * <=256 fine steps, FP64 intersection. */
 void mm_march_params_scenario(mm_march_params* p);
 
+/* Hard bound on march loop trips per ray (counted or not). The reference's uncounted `continue`
+ * paths (comp:433, 437) are only bounded by t reaching the outer shell; this cap keeps every ray
+ * finite for degenerate parameters. Never reached by the presets. */
+#define MM_MAX_MARCH_ITERATIONS 16384
+
 #define MM_PHASE_ALL (-1)       /* march every pixel */
 #define MM_PHASE_FROM_UBO (-2)  /* phase = int(sun.color.a) like the reference (comp:292) */
 

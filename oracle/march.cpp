@@ -387,7 +387,9 @@ void marchPixel(const Scene<T>& s, int px, int py, T out[4], int* cnt) {
     const V3<T> windH = v3<T>(T(0.1), T(0.05), T(0));
     const T HG = fmaxT(hgPhase<T>(cosTheta, T(0.6)), T(0.7) * hgPhase<T>(cosTheta, T(0.99) - T(0.1)));
 
+    int iterations = 0;
     for (T t = tStart; t < tEnd; t += stepSize) {
+        if (++iterations > MM_MAX_MARCH_ITERATIONS) break;  // same hard bound as the GPU kernel
         const int mode = noHits ? 0 : 1;
         if (bounded && modeLimit[mode] >= 0 && ++modeCount[mode] > modeLimit[mode]) break;
         V3<T> currentPos = cameraPos + (t * rd);

@@ -17,7 +17,7 @@ import os
 from dataclasses import dataclass
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmm.so")
+LIB_PATH = os.environ.get("MM_LIBRARY", os.path.join(_HERE, "libmm.so"))
 
 # ---------------------------------------------------------------- constants (include/mm.h)
 MM_OK = 0

@@ -10,8 +10,8 @@
 //   FSET_FAST: weather RG8, curl RGBA8, low-res R8 + 16-bit erosion sum, hi-res 16-bit erosion sum,
 //              apron layout (N+1 texels per axis) so a filter footprint never wraps
 //   FSET_PREC: weather RG8, curl FP16x4, low-res FP8 e4m3 x4 with per-channel scale, hi-res FP16x4
-// Built with -ffp-contract=off and GLSL-grade division/sqrt (approximate reciprocal): every other
-// float operation rounds as the oracle's restatement does.
+// Built with -ffp-contract=off and IEEE division/sqrt: apart from the filter lerps (FMA) and pow/exp
+// (hardware exp2/log2, GLSL precision), every float operation rounds as the oracle's restatement does.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include "mm_internal.hpp"
@@ -37,13 +37,27 @@ __device__ __forceinline__ V3 operator*(float s, V3 a) { return mk(s * a.x, s * 
 __device__ __forceinline__ V3 mul3(V3 a, V3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
 __device__ __forceinline__ float dot3(V3 a, V3 b) { return ((a.x * b.x) + (a.y * b.y)) + (a.z * b.z); }
 __device__ __forceinline__ V3 ld3(const float* f) { return mk(f[0], f[1], f[2]); }
-__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
-__device__ __forceinline__ V3 normalize3(V3 a) { return a * rsq(dot3(a, a)); }
-__device__ __forceinline__ float length3(V3 a) { return sqrt_(dot3(a, a)); }
+// normalize with a correctly rounded 1 / sqrt: getProjectedShellPoint scales the unit vector by the
+// shell radius (6.4e6 m in the Earth-radius scenario), so one ulp of the reciprocal moves the
+// projected point by ~0.5 m = 2e-4 of the shell height and flips density thresholds downstream.
+__device__ __forceinline__ V3 normalize3(V3 a) { return a * __fdiv_rn(1.0f, __fsqrt_rn(dot3(a, a))); }
+__device__ __forceinline__ float length3(V3 a) { return __fsqrt_rn(dot3(a, a)); }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
-__device__ __forceinline__ float lerpf(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
+#ifndef MM_LERP_FMA
+#define MM_LERP_FMA 1
+#endif
+#ifndef MM_POW_ACCURATE
+#define MM_POW_ACCURATE 0
+#endif
+__device__ __forceinline__ float lerpf(float a, float b, float t) {
+#if MM_LERP_FMA
+    return __builtin_fmaf(t, b - a, a);
+#else
+    return a + t * (b - a);
+#endif
+}
 __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
     float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
     return t * t * (3.0f - 2.0f * t);
@@ -51,8 +65,13 @@ __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
 __device__ __forceinline__ float exp2_(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float log2_(float x) { return __builtin_amdgcn_logf(x); }
 // GLSL-grade pow/exp: exp2(y * log2(x)), exp2(x * log2(e)).
+#if MM_POW_ACCURATE
+__device__ __forceinline__ float powg(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ float expg(float x) { return expf(x); }
+#else
 __device__ __forceinline__ float powg(float x, float y) { return exp2_(y * log2_(x)); }
 __device__ __forceinline__ float expg(float x) { return exp2_(x * 1.4426950408889634f); }
+#endif
 // remap with a constant range: (v - a) * (1 / (b - a)) * (d - c) + c
 __device__ __forceinline__ float remapk(float v, float a, float inv_ba, float c, float dc) {
     return c + (((v - a) * inv_ba) * dc);
@@ -370,24 +389,30 @@ __device__ V3 atmosphere(const MarchConsts& c, V3 dir, V3 sunDir) {
     return out;
 }
 
+// IEEE division / square root in the intersection's arithmetic type.
+__device__ __forceinline__ float div_(float a, float b) { return __fdiv_rn(a, b); }
+__device__ __forceinline__ double div_(double a, double b) { return a / b; }
+__device__ __forceinline__ float sqrtx(float a) { return __fsqrt_rn(a); }
+__device__ __forceinline__ double sqrtx(double a) { return sqrt(a); }
+
 // raySphereIntersection (comp:147-177) in arithmetic type I; a miss yields t = 0.
 template <class I>
 __device__ __forceinline__ I sphere_t(I rox, I roy, I roz, I rdx, I rdy, I rdz, I cx, I cy, I cz, I w, bool& valid) {
-    rox = (rox - cx) / w; roy = (roy - cy) / w; roz = (roz - cz) / w;
+    rox = div_(rox - cx, w); roy = div_(roy - cy, w); roz = div_(roz - cz, w);
     I A = (rdx * rdx + rdy * rdy) + rdz * rdz;
     I B = I(2) * ((rdx * rox + rdy * roy) + rdz * roz);
     I C = ((rox * rox + roy * roy) + roz * roz) - I(0.25);
     I disc = (B * B) - ((I(4) * A) * C);
     valid = false;
     if (disc < I(0)) return I(0);
-    I sq = sqrt(disc);
-    I t = ((-sq - B) / A) * I(0.5);
-    if (t < I(0)) t = ((sq - B) / A) * I(0.5);
+    I sq = sqrtx(disc);
+    I t = div_(-sq - B, A) * I(0.5);
+    if (t < I(0)) t = div_(sq - B, A) * I(0.5);
     if (!(t >= I(0))) return I(0);
     valid = true;
     I px = (rox + rdx * t) * w + cx, py = (roy + rdy * t) * w + cy, pz = (roz + rdz * t) * w + cz;
     I dx = px - rox, dy = py - roy, dz = pz - roz;
-    return sqrt((dx * dx + dy * dy) + dz * dz);
+    return sqrtx((dx * dx + dy * dy) + dz * dz);
 }
 
 __device__ __forceinline__ float pow_chain4(float d) { return d * ((d * d) * d); }
@@ -441,7 +466,8 @@ __global__ __launch_bounds__(256) void march_kernel(const MarchConsts c, const M
     int nd = 0, nh = 0;
 
     // ray generation (comp:303-322)
-    float uvx = (float)px / (float)c.width, uvy = (float)py / (float)c.height;
+    // IEEE division: the pixel's uv feeds the ray direction, whose ulps the Earth-radius shell amplifies
+    float uvx = __fdiv_rn((float)px, (float)c.width), uvy = __fdiv_rn((float)py, (float)c.height);
     float sx = uvx * 2.0f - 1.0f, sy = uvy * 2.0f - 1.0f;
     const V3 camPos = ld3(c.camPos);
     V3 p = (ld3(c.refPoint) + (((c.aspect * sx) * c.tanH) * ld3(c.camRight))) - ((sy * c.tanH) * ld3(c.camUp));
@@ -524,7 +550,11 @@ __global__ __launch_bounds__(256) void march_kernel(const MarchConsts c, const M
     int misses = 0, steps = 0, modeCount0 = 0, modeCount1 = 0;
     const bool bounded = (c.flags & MM_MARCH_BOUNDED) != 0;
 
+    int iterations = 0;
     for (float t = tStart; t < tEnd; t += stepSize) {
+        // hard bound on loop trips (MM_MAX_MARCH_ITERATIONS): guarantees termination even when
+        // t += stepSize stops advancing t in float for degenerate parameters
+        if (++iterations > MM_MAX_MARCH_ITERATIONS) break;
         const bool coarse = noHits;
         if (bounded) {
             if (coarse) { if (c.maxCoarse >= 0 && ++modeCount0 > c.maxCoarse) break; }
@@ -569,7 +599,7 @@ __global__ __launch_bounds__(256) void march_kernel(const MarchConsts c, const M
             misses++;
             if (misses >= c.missLimit) {
                 noHits = true;
-                stepSize /= c.fineMul;
+                stepSize = __fdiv_rn(stepSize, c.fineMul);  // IEEE: every later sample position depends on it
             }
         }
         if (accum > c.opacityCutoff) {
