@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--fset", default="fast", choices=["fast", "ref", "prec"])
+    ap.add_argument("--no-batch", action="store_true", help="evaluate light cones in place (lock-step kernel)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device-to-device copies of MIB MiB (HBM counter calibration)")
@@ -123,6 +124,8 @@ def main():
     cam, sun, sky = mm.build_ubos(mm.scenario_scene(altitude=200.0, pitch_deg=20.0, fov_deg=60.0, aspect=W / H,
                                                     sun_elevation_deg=25.0, time=0.0))
     prm = mm.march_params("scenario")
+    if args.no_batch:
+        prm.flags |= mm.MARCH_NO_BATCH
     ns = mm.NoiseSet(args.fset)
     nparams = mm.noise_params(seed=1, weather_seed=1, coverage=0.5)
     out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
@@ -182,7 +185,8 @@ def main():
         "config": {"workload": f"{W}x{H} scenario frame: all noise textures regenerated, then full-frame FP32 march "
                                f"(Earth radius 6.371e6 m, shell 1500-4000 m, camera 200 m pitched +20 deg, 60 deg "
                                f"FOV, sun 25 deg, coverage 0.5, t = 0, weather seed 1)",
-                   "width": W, "height": H, "format_set": args.fset, "coarse_steps_max": 128, "fine_steps_max": 256},
+                   "width": W, "height": H, "format_set": args.fset,
+                   "kernel": "lock-step" if args.no_batch else "cone-batched", "coarse_steps_max": 128, "fine_steps_max": 256},
         "ms_per_frame": round(ms_step, 4),
         "frames_per_s": round(1000.0 / ms_step, 2),
         "march_ms": round(march_ms, 4),

@@ -154,7 +154,7 @@ enum {
     MM_MARCH_ISECT_FP64 = 1u << 0,   /* ray/shell intersection in FP64 (same formula as comp:147-177) */
     MM_MARCH_BOUNDED = 1u << 1,      /* every fine-mode sample counts toward max_fine_steps (the
                                         reference's `continue` at comp:437 does not count) */
-    MM_MARCH_NO_COMPACT = 1u << 2    /* disable wave-level compaction of terminated rays */
+    MM_MARCH_NO_BATCH = 1u << 2      /* evaluate light cones in place (no wave-level cone batching) */
 };
 
 enum { MM_CONE_PLAIN = 0, MM_CONE_SURROGATE = 1 };

@@ -30,7 +30,7 @@ MM_E_NULL, MM_E_DIM, MM_E_PHASE, MM_E_PITCH, MM_E_FORMAT, MM_E_PARAM, MM_E_DEVIC
 FMT_RGBA8, FMT_RG8, FMT_R8E16, FMT_E16, FMT_FP8X4, FMT_FP16X4 = 1, 2, 3, 4, 5, 6
 LAYOUT_LINEAR, LAYOUT_APRON = 0, 1
 GEN_WEATHER, GEN_CURL, GEN_LOW_RES, GEN_HI_RES, GEN_ALL = 1, 2, 4, 8, 15
-MARCH_ISECT_FP64, MARCH_BOUNDED, MARCH_NO_COMPACT = 1, 2, 4
+MARCH_ISECT_FP64, MARCH_BOUNDED, MARCH_NO_BATCH = 1, 2, 4
 CONE_PLAIN, CONE_SURROGATE = 0, 1
 PHASE_ALL, PHASE_FROM_UBO = -1, -2
 TEXEL_BYTES = {FMT_RGBA8: 4, FMT_RG8: 2, FMT_R8E16: 4, FMT_E16: 2, FMT_FP8X4: 4, FMT_FP16X4: 8}

@@ -355,7 +355,9 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
         c.gridH = height;
     }
     if (c.gridW <= 0 || c.gridH <= 0) return MM_OK;  // no pixel of this phase in the image
-    c.tilesX = (c.gridW + 15) / 16;
+    c.tilesX = (c.gridW + 7) / 8;
+    c.tilesY = (c.gridH + 7) / 8;
+    c.tilesPerXcd = (c.tilesX * c.tilesY + 7) / 8;
 
     mm::MarchTextures t;
     memset(&t, 0, sizeof(t));

@@ -76,8 +76,12 @@ __device__ __forceinline__ float expg(float x) { return exp2_(x * 1.442695040888
 __device__ __forceinline__ float remapk(float v, float a, float inv_ba, float c, float dc) {
     return c + (((v - a) * inv_ba) * dc);
 }
-__device__ __forceinline__ float remapClamped01(float v, float a) {  // remapClamped(v, a, 1, 0, 1)
-    return clampf(0.0f + (((v - a) / (1.0f - a)) * 1.0f), 0.0f, 1.0f);
+// remapClamped(v, a, 1, 0, 1) for a in [0, 1]. The IEEE quotient is only needed when v > a: otherwise
+// (v - a) / (1 - a) <= 0 (or 0/0 when v = a = 1) and the clamp yields 0 exactly, so the division
+// sits behind a branch that whole waves of empty samples skip.
+__device__ __forceinline__ float remapClamped01(float v, float a) {
+    if (!(v > a)) return 0.0f;
+    return clampf(0.0f + ((__fdiv_rn(v - a, 1.0f - a)) * 1.0f), 0.0f, 1.0f);
 }
 
 // ------------------------------------------------------------------ sampler
@@ -320,10 +324,13 @@ __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTex
     if (COUNT) nd++;
     const int q8 = c.subtexelBits;
     V3 proj = proj_shell(c, pos, center);
-    LowG<FSET> lg = gather_low<FSET>(tx.low, pos * c.lowScale, q8);
     Weather w = read_weather<FSET>(tx.weather, c.weatherScale * (proj.x - c.camPos[0]),
                                    c.weatherScale * (proj.z - c.camPos[2]), q8);
     float layer = layer_density(h, w.type);
+    // density = layer * remapClamped(R, ...) with the remap in [0, 1]: a zero layer density gives 0
+    // whatever the low-res texel, so its gather is skipped (comp:240-243 evaluate to the same 0)
+    if (layer == 0.0f) return 0.0f;
+    LowG<FSET> lg = gather_low<FSET>(tx.low, pos * c.lowScale, q8);
     float shape = low_shape<FSET>(lg, tx.low);
     float density = layer * clampf(0.0f + (((shape - 0.3f) * (1.0f / (1.0f - 0.3f))) * 1.0f), 0.0f, 1.0f);
     if (density < 0.0001f) return 0.0f;
@@ -445,33 +452,25 @@ __device__ V3 night_sky(const MarchConsts& c, const MarchTextures& tx, float u, 
     return mk(filt2(g, Byte<0>()) * kInv255, filt2(g, Byte<1>()) * kInv255, filt2(g, Byte<2>()) * kInv255);
 }
 
-// ------------------------------------------------------------------ kernel
-template <int FSET, bool COUNT>
-__global__ __launch_bounds__(256) void march_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
-                                                    unsigned long long* __restrict__ counts) {
-    // block = 16x16 marched pixels as 4 waves of 8x8
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tileX = blockIdx.x % c.tilesX, tileY = blockIdx.x / c.tilesX;
-    const int gi = tileX * 16 + (wave & 1) * 8 + (lane & 7);
-    const int gj = tileY * 16 + (wave >> 1) * 8 + (lane >> 3);
-    if (gi >= c.gridW || gj >= c.gridH) return;
-    int px, py;
-    if (c.phase >= 0) {
-        px = gi * 4 + (c.phase & 3);
-        py = gj * 4 + (c.phase >> 2);
-    } else {
-        px = gi;
-        py = gj;
-    }
-    int nd = 0, nh = 0;
+// ------------------------------------------------------------------ per-ray pieces
+// Everything a ray needs before and after its march (comp:303-384, 484-499).
+struct RaySetup {
+    V3 rd, bg;
+    float alpha;        // finalColor.a before the opacity factor
+    float tStart, tEnd;
+    float cosTheta, HG, beerBlend;
+    bool night, march;  // march = false: the sky colour was already stored (comp:351-354)
+};
 
-    // ray generation (comp:303-322)
-    // IEEE division: the pixel's uv feeds the ray direction, whose ulps the Earth-radius shell amplifies
+__device__ __forceinline__ RaySetup ray_setup(const MarchConsts& c, const MarchTextures& tx, int px, int py) {
+    RaySetup r;
+    // ray generation (comp:303-322); IEEE division: the pixel's uv feeds the ray direction
     float uvx = __fdiv_rn((float)px, (float)c.width), uvy = __fdiv_rn((float)py, (float)c.height);
     float sx = uvx * 2.0f - 1.0f, sy = uvy * 2.0f - 1.0f;
     const V3 camPos = ld3(c.camPos);
     V3 p = (ld3(c.refPoint) + (((c.aspect * sx) * c.tanH) * ld3(c.camRight))) - ((sy * c.tanH) * ld3(c.camUp));
-    const V3 rd = normalize3(p - camPos);
+    r.rd = normalize3(p - camPos);
+    const V3 rd = r.rd;
     const V3 sunDir = ld3(c.sunDir);
 
     // sun glow / alpha (comp:324-340)
@@ -482,54 +481,46 @@ __global__ __launch_bounds__(256) void march_kernel(const MarchConsts c, const M
     float sunDisk = smoothstepf(SUN_COS, SUN_COS + 0.00003f, dotToSun);
     dotToSun = pow_chain4(pow_chain4(pow_chain4(pow_chain4(dotToSun))));
     dotToSun = dotToSun * (dotToSun * dotToSun);
-    const bool night = c.sunDirY < 0.0f;
-    if (night) {
+    r.night = c.sunDirY < 0.0f;
+    if (r.night) {
         float d = dotToSun;
         dotToSun = d * ((((((d * d) * d) * d) * d) * d) * d);
     }
     sunDisk = fmaxf(0.0f, fmaxf(sunDisk, dotToSun));
-
-    float4 fc = make_float4(0.f, 0.f, 0.f, 0.f);
-    V3 bg = mk(0.f, 0.f, 0.f);
-    if (!night) {
-        bg = atmosphere(c, rd, sunDir);
-        fc = make_float4(bg.x, bg.y, bg.z, fmaxf(skyAmbient, sunDisk));
+    r.bg = mk(0.f, 0.f, 0.f);
+    r.alpha = 0.0f;
+    if (!r.night) {
+        r.bg = atmosphere(c, rd, sunDir);
+        r.alpha = fmaxf(skyAmbient, sunDisk);
     }
-    float4* dst = reinterpret_cast<float4*>(out) + ((size_t)py * c.pitch + px);
-    if (rd.y < 0.0f) {  // comp:351-354
-        *dst = fc;
-        return;
-    }
+    r.march = !(rd.y < 0.0f);
+    r.tStart = r.tEnd = 0.0f;
+    r.cosTheta = r.HG = r.beerBlend = 0.0f;
+    if (!r.march) return r;
 
     V3 center = camPos;
     center.y = c.earthCenter[1];
-    float tStart, tEnd;
-    {
-        bool vi, vo;
-        if (c.flags & MM_MARCH_ISECT_FP64) {
-            double ti = sphere_t<double>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
-                                         (double)c.innerDiam, vi);
-            double to = sphere_t<double>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
+    bool vi, vo;
+    if (c.flags & MM_MARCH_ISECT_FP64) {
+        r.tStart = (float)sphere_t<double>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
+                                           (double)c.innerDiam, vi);
+        r.tEnd = (float)sphere_t<double>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
                                          (double)c.outerDiam, vo);
-            tStart = (float)ti;
-            tEnd = (float)to;
-        } else {
-            tStart = sphere_t<float>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
-                                     c.innerDiam, vi);
-            tEnd = sphere_t<float>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
-                                   c.outerDiam, vo);
-        }
+    } else {
+        r.tStart = sphere_t<float>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
+                                   c.innerDiam, vi);
+        r.tEnd = sphere_t<float>(camPos.x, camPos.y, camPos.z, rd.x, rd.y, rd.z, center.x, center.y, center.z,
+                                 c.outerDiam, vo);
     }
-
-    if (night) {  // comp:365-384
+    if (r.night) {  // comp:365-384
         V3 c0 = ld3(c.nightRot[0]), c1 = ld3(c.nightRot[1]), c2 = ld3(c.nightRot[2]);
         V3 rrd = ((c0 * rd.x) + (c1 * rd.y)) + (c2 * rd.z);
         V3 rro = ((c0 * camPos.x) + (c1 * camPos.y)) + (c2 * camPos.z);
-        V3 point = (rrd * tEnd) + rro;
+        V3 point = (rrd * r.tEnd) + rro;
         V3 pr = proj_shell(c, point, center);
         float u = 0.00002f * (pr.x - camPos.x) + 0.35f;
         float v = 0.00002f * (pr.z - camPos.z) + 0.35f;
-        bg = night_sky(c, tx, u, v);
+        V3 bg = night_sky(c, tx, u, v);
         bg = mk(bg.x * (sqrt_(bg.x) * 0.75f), bg.y * (sqrt_(bg.y) * 0.75f), bg.z * (sqrt_(bg.z) * 0.75f));
         bg = mk(powg(bg.x, 2.2f), powg(bg.y, 2.2f), powg(bg.z, 2.2f));
         bg = bg * 10.0f;
@@ -537,101 +528,283 @@ __global__ __launch_bounds__(256) void march_kernel(const MarchConsts c, const M
         bg = bg * falloff;
         float pw = powg(rd.y, 0.03125f);
         bg = mk(mixf(0.3f * 0.05f, bg.x, pw), mixf(0.6f * 0.05f, bg.y, pw), mixf(4.0f * 0.05f, bg.z, pw));
-        bg = bg + mk(sunDisk, sunDisk, sunDisk);
-        fc.w = sunDisk;
+        r.bg = bg + mk(sunDisk, sunDisk, sunDisk);
+        r.alpha = sunDisk;
     }
+    r.cosTheta = dot3(rd, sunDir);
+    r.HG = fmaxf(hg_phase(r.cosTheta, 0.6f), 0.7f * hg_phase(r.cosTheta, 0.99f - 0.1f));
+    r.beerBlend = (-r.cosTheta) * 0.5f + 0.5f;
+    return r;
+}
 
-    // march (comp:386-482)
-    const float cosTheta = dot3(rd, sunDir);
-    float accum = 0.0f, transmittance = 1.0f;
-    float stepSize = c.stepSize0;
-    const float HG = fmaxf(hg_phase(cosTheta, 0.6f), 0.7f * hg_phase(cosTheta, 0.99f - 0.1f));
-    bool noHits = true;
-    int misses = 0, steps = 0, modeCount0 = 0, modeCount1 = 0;
-    const bool bounded = (c.flags & MM_MARCH_BOUNDED) != 0;
-
-    int iterations = 0;
-    for (float t = tStart; t < tEnd; t += stepSize) {
-        // hard bound on loop trips (MM_MAX_MARCH_ITERATIONS): guarantees termination even when
-        // t += stepSize stops advancing t in float for degenerate parameters
-        if (++iterations > MM_MAX_MARCH_ITERATIONS) break;
-        const bool coarse = noHits;
-        if (bounded) {
-            if (coarse) { if (c.maxCoarse >= 0 && ++modeCount0 > c.maxCoarse) break; }
-            else { if (c.maxFine >= 0 && ++modeCount1 > c.maxFine) break; }
-        }
-        V3 currentPos = camPos + (t * rd);
-        V3 currentProj = proj_shell(c, currentPos, center);
-        float rHeight = rel_height(c, currentPos, currentProj);
-        V3 wo = wind_offset(c, rHeight);
-        float density = cloud_test<FSET, COUNT>(c, tx, currentPos + wo, rHeight, center, nd);
-        float loDensity = density;
-        if (density > 0.0f) {
-            misses = 0;
-            if (noHits) {
-                t -= stepSize;
-                stepSize *= c.fineMul;
-                noHits = false;
-                continue;
-            }
-            density = cloud_hires<FSET, COUNT>(c, tx, currentPos + wo, stepSize, density, rHeight, nh);
-            if (density < 0.0001f) continue;
-            float dal = 0.0f;
-#pragma unroll 1
-            for (int i = 0; i < 6; i++) {
-                V3 lsPos = currentPos + ((3.0f * stepSize) * ld3(c.coneOffsets[i]));
-                V3 lsProj = proj_shell(c, lsPos, center);
-                float lsHeight = rel_height(c, lsPos, lsProj);
-                V3 lwo = wind_offset(c, lsHeight);
-                float ls = cloud_test<FSET, COUNT>(c, tx, lsPos + lwo, lsHeight, center, nd);
-                if (ls > 0.0f) dal += cloud_hires<FSET, COUNT>(c, tx, lsPos + lwo, stepSize, ls, lsHeight, nh);
-            }
-            float beers = expg(-dal);
-            float beersMod = fmaxf(beers, 0.7f * expg(-0.25f * dal));
-            beers = mixf(beers, beersMod, (-cosTheta) * 0.5f + 0.5f);
-            float e1 = clampf(remapk(rHeight, 0.3f, 1.0f / (0.85f - 0.3f), 0.5f, 2.0f - 0.5f), 0.5f, 2.0f);
-            float inScatter = 0.09f + powg(loDensity, e1);
-            float e2 = clampf(remapk(rHeight, 0.07f, 1.0f / (0.34f - 0.07f), 0.1f, 1.0f - 0.1f), 0.1f, 1.0f);
-            inScatter *= powg(e2, 0.8f);
-            transmittance = mixf(transmittance, (inScatter * HG) * beers, 1.0f - accum);
-            accum += density;
-        } else if (!noHits) {
-            misses++;
-            if (misses >= c.missLimit) {
-                noHits = true;
-                stepSize = __fdiv_rn(stepSize, c.fineMul);  // IEEE: every later sample position depends on it
-            }
-        }
-        if (accum > c.opacityCutoff) {
-            accum = 1.0f;
-            break;
-        }
-        if (c.maxSteps >= 0 && ++steps > c.maxSteps) break;
-        if (!bounded) {
-            if (coarse) { if (c.maxCoarse >= 0 && ++modeCount0 > c.maxCoarse) break; }
-            else { if (c.maxFine >= 0 && ++modeCount1 > c.maxFine) break; }
-        }
-    }
-
-    // compose (comp:484-499)
-    accum *= smoothstepf(0.0f, 1.0f, fminf(1.0f, remapk(rd.y, 0.0f, 1.0f / 0.1f, 0.0f, 1.0f)));
+// compose (comp:484-499)
+__device__ __forceinline__ float4 compose(const MarchConsts& c, const RaySetup& r, float accum, float transmittance) {
+    accum *= smoothstepf(0.0f, 1.0f, fminf(1.0f, remapk(r.rd.y, 0.0f, 1.0f / 0.1f, 0.0f, 1.0f)));
     accum = fminf(accum, 0.999f);
     const float I = c.sunIntensity;
     const float e = expg(-transmittance);
     V3 amb;
-    if (!night) amb = (bg * 0.08f) * e;
+    if (!r.night) amb = (r.bg * 0.08f) * e;
     else {
-        float pw = powg(rd.y, 0.03125f);
+        float pw = powg(r.rd.y, 0.03125f);
         amb = (mk(0.3f * 0.05f, 0.6f * 0.05f, 4.0f * 0.05f) * pw * 0.08f) * e;
     }
     const float direct = I * fmaxf(0.0f, transmittance);
     V3 cloud = mk(c.sunColor[0] * (direct + amb.x), c.sunColor[1] * (direct + amb.y), c.sunColor[2] * (direct + amb.z));
     float4 res;
-    res.x = mixf(bg.x, cloud.x, accum);
-    res.y = mixf(bg.y, cloud.y, accum);
-    res.z = mixf(bg.z, cloud.z, accum);
-    res.w = fc.w * fmaxf(1.0f - accum, 0.0f);
-    *dst = res;
+    res.x = mixf(r.bg.x, cloud.x, accum);
+    res.y = mixf(r.bg.y, cloud.y, accum);
+    res.z = mixf(r.bg.z, cloud.z, accum);
+    res.w = r.alpha * fmaxf(1.0f - accum, 0.0f);
+    return res;
+}
+
+// Density along the sun: the six-sample light cone (comp:440-453).
+template <int FSET, bool COUNT>
+__device__ __forceinline__ float cone_dal(const MarchConsts& c, const MarchTextures& tx, V3 currentPos, float stepSize,
+                                          int& nd, int& nh) {
+    V3 center = ld3(c.camPos);
+    center.y = c.earthCenter[1];
+    float dal = 0.0f;
+#pragma unroll 1
+    for (int i = 0; i < 6; i++) {
+        V3 lsPos = currentPos + ((3.0f * stepSize) * ld3(c.coneOffsets[i]));
+        V3 lsProj = proj_shell(c, lsPos, center);
+        float lsHeight = rel_height(c, lsPos, lsProj);
+        V3 lwo = wind_offset(c, lsHeight);
+        float ls = cloud_test<FSET, COUNT>(c, tx, lsPos + lwo, lsHeight, center, nd);
+        if (ls > 0.0f) dal += cloud_hires<FSET, COUNT>(c, tx, lsPos + lwo, stepSize, ls, lsHeight, nh);
+    }
+    return dal;
+}
+
+// Beer's law with the artistic floor, as a multiplier of the in-scatter term (comp:456-460).
+__device__ __forceinline__ float beers_of(float dal, float beerBlend) {
+    float beers = expg(-dal);
+    float beersMod = fmaxf(beers, 0.7f * expg(-0.25f * dal));
+    return mixf(beers, beersMod, beerBlend);
+}
+
+// in-scatter x phase of a fine hit (comp:461-462, 464): the factor that multiplies Beer's law
+__device__ __forceinline__ float inscatter_hg(float loDensity, float rHeight, float HG) {
+    float e1 = clampf(remapk(rHeight, 0.3f, 1.0f / (0.85f - 0.3f), 0.5f, 2.0f - 0.5f), 0.5f, 2.0f);
+    float inScatter = 0.09f + powg(loDensity, e1);
+    float e2 = clampf(remapk(rHeight, 0.07f, 1.0f / (0.34f - 0.07f), 0.1f, 1.0f - 0.1f), 0.1f, 1.0f);
+    inScatter *= powg(e2, 0.8f);
+    return inScatter * HG;
+}
+
+// State of one ray's march loop (comp:386-482) between loop trips.
+struct MarchState {
+    float t, stepSize, accum;
+    int misses, steps, mc0, mc1, iterations;
+    bool noHits, active;
+};
+
+__device__ __forceinline__ void march_init(const MarchConsts& c, const RaySetup& r, MarchState& m) {
+    m.t = r.tStart;
+    m.stepSize = c.stepSize0;
+    m.accum = 0.0f;
+    m.misses = m.steps = m.mc0 = m.mc1 = m.iterations = 0;
+    m.noHits = true;
+    m.active = r.march && (m.t < r.tEnd);
+}
+
+// One trip of the reference's for-loop, including its increment. A fine-mode hit that survives the
+// hi-res erosion returns a light-cone job (position and step size at this trip, the in-scatter
+// factor and the mix weight 1 - accum) instead of evaluating the cone in place; its result only
+// enters the transmittance recurrence, never the loop's control flow.
+template <int FSET, bool COUNT>
+__device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchTextures& tx, const RaySetup& r,
+                                           MarchState& m, V3& jobPos, float& jobStep, float& jobX0, float& jobW,
+                                           int& nd, int& nh) {
+    bool job = false;
+    // hard bound on loop trips: guarantees termination even if t += stepSize stops advancing t
+    if (++m.iterations > MM_MAX_MARCH_ITERATIONS) { m.active = false; return false; }
+    const bool coarse = m.noHits;
+    const bool bounded = (c.flags & MM_MARCH_BOUNDED) != 0;
+    if (bounded) {
+        if (coarse) { if (c.maxCoarse >= 0 && ++m.mc0 > c.maxCoarse) { m.active = false; return false; } }
+        else { if (c.maxFine >= 0 && ++m.mc1 > c.maxFine) { m.active = false; return false; } }
+    }
+    const V3 camPos = ld3(c.camPos);
+    V3 center = camPos;
+    center.y = c.earthCenter[1];
+    V3 currentPos = camPos + (m.t * r.rd);
+    V3 currentProj = proj_shell(c, currentPos, center);
+    float rHeight = rel_height(c, currentPos, currentProj);
+    V3 wo = wind_offset(c, rHeight);
+    float density = cloud_test<FSET, COUNT>(c, tx, currentPos + wo, rHeight, center, nd);
+    float loDensity = density;
+    bool cont = false;  // `continue`: skip the loop tail but still run the increment
+    if (density > 0.0f) {
+        m.misses = 0;
+        if (m.noHits) {
+            m.t -= m.stepSize;
+            m.stepSize *= c.fineMul;
+            m.noHits = false;
+            cont = true;
+        } else {
+            density = cloud_hires<FSET, COUNT>(c, tx, currentPos + wo, m.stepSize, density, rHeight, nh);
+            if (density < 0.0001f) {
+                cont = true;
+            } else {
+                job = true;
+                jobPos = currentPos;
+                jobStep = m.stepSize;
+                jobX0 = inscatter_hg(loDensity, rHeight, r.HG);
+                jobW = 1.0f - m.accum;
+                m.accum += density;
+            }
+        }
+    } else if (!m.noHits) {
+        m.misses++;
+        if (m.misses >= c.missLimit) {
+            m.noHits = true;
+            m.stepSize = __fdiv_rn(m.stepSize, c.fineMul);  // IEEE: every later sample position depends on it
+        }
+    }
+    if (!cont) {
+        if (m.accum > c.opacityCutoff) { m.accum = 1.0f; m.active = false; return job; }
+        if (c.maxSteps >= 0 && ++m.steps > c.maxSteps) { m.active = false; return job; }
+        if (!bounded) {
+            if (coarse) { if (c.maxCoarse >= 0 && ++m.mc0 > c.maxCoarse) { m.active = false; return job; } }
+            else { if (c.maxFine >= 0 && ++m.mc1 > c.maxFine) { m.active = false; return job; } }
+        }
+    }
+    m.t += m.stepSize;
+    if (!(m.t < r.tEnd)) m.active = false;
+    return job;
+}
+
+// Tile order: the hardware deals workgroups round-robin to the 8 XCDs (block b -> XCD b mod 8), so
+// block b takes tile (b mod 8) * per + b / 8 and every XCD marches a contiguous band of tiles whose
+// texel footprint stays in its own L2.
+__device__ __forceinline__ int xcd_tile(int b, int per) { return (b & 7) * per + (b >> 3); }
+
+__device__ __forceinline__ void tile_pixel(const MarchConsts& c, int gi, int gj, int& px, int& py) {
+    if (c.phase >= 0) {
+        px = gi * 4 + (c.phase & 3);
+        py = gj * 4 + (c.phase >> 2);
+    } else {
+        px = gi;
+        py = gj;
+    }
+}
+
+// ------------------------------------------------------------------ lock-step kernel
+// One wave per 8x8 tile; every lane evaluates its light cones in place (the shader's structure).
+template <int FSET, bool COUNT>
+__global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
+                                                   unsigned long long* __restrict__ counts) {
+    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
+    if (tile >= c.tilesX * c.tilesY) return;
+    const int lane = threadIdx.x;
+    const int gi = (tile % c.tilesX) * 8 + (lane & 7), gj = (tile / c.tilesX) * 8 + (lane >> 3);
+    if (gi >= c.gridW || gj >= c.gridH) return;
+    int px, py;
+    tile_pixel(c, gi, gj, px, py);
+    int nd = 0, nh = 0;
+    const RaySetup r = ray_setup(c, tx, px, py);
+    float4* dst = reinterpret_cast<float4*>(out) + ((size_t)py * c.pitch + px);
+    if (!r.march) {
+        *dst = make_float4(r.bg.x, r.bg.y, r.bg.z, r.alpha);
+        return;
+    }
+    MarchState m;
+    march_init(c, r, m);
+    float transmittance = 1.0f;
+    while (m.active) {
+        V3 jp;
+        float js, jx0, jw;
+        if (march_trip<FSET, COUNT>(c, tx, r, m, jp, js, jx0, jw, nd, nh)) {
+            float dal = cone_dal<FSET, COUNT>(c, tx, jp, js, nd, nh);
+            transmittance = mixf(transmittance, jx0 * beers_of(dal, r.beerBlend), jw);
+        }
+    }
+    *dst = compose(c, r, m.accum, transmittance);
+    if (COUNT) {
+        atomicAdd(&counts[0], (unsigned long long)nd);
+        atomicAdd(&counts[1], (unsigned long long)nh);
+    }
+}
+
+// ------------------------------------------------------------------ cone-batched kernel
+// One wave per 8x8 tile. Lanes run their primary march in lock-step but hand light-cone work to a
+// wave-local LDS queue (slots assigned with ballot + mbcnt in creation order); once 64 jobs are
+// queued (or no lane is still marching) the wave evaluates a full batch, one job per lane, and each
+// lane folds its results into its transmittance in creation order. Cone work therefore runs at
+// full SIMD width instead of waiting on the few lanes that are inside a cloud, and every lane
+// performs exactly the shader's operations in the shader's order.
+constexpr int kQueueCap = 128;
+
+struct ConeQueue {
+    float x[kQueueCap], y[kQueueCap], z[kQueueCap], step[kQueueCap];
+    float x0[kQueueCap], w[kQueueCap], blend[kQueueCap], res[kQueueCap];
+    int owner[kQueueCap];
+};
+
+template <int FSET, bool COUNT>
+__global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, const MarchTextures tx,
+                                                           float* __restrict__ out,
+                                                           unsigned long long* __restrict__ counts) {
+    __shared__ ConeQueue q;
+    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
+    if (tile >= c.tilesX * c.tilesY) return;  // uniform across the block
+    const int lane = threadIdx.x;
+    const int gi = (tile % c.tilesX) * 8 + (lane & 7), gj = (tile / c.tilesX) * 8 + (lane >> 3);
+    const bool valid = gi < c.gridW && gj < c.gridH;
+    int px = 0, py = 0;
+    if (valid) tile_pixel(c, gi, gj, px, py);
+    int nd = 0, nh = 0;
+    RaySetup r;
+    MarchState m;
+    m.active = false;
+    if (valid) {
+        r = ray_setup(c, tx, px, py);
+        if (!r.march)
+            reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = make_float4(r.bg.x, r.bg.y, r.bg.z, r.alpha);
+        march_init(c, r, m);
+    }
+    float transmittance = 1.0f;
+    int qn = 0;  // queued jobs (wave-uniform)
+    for (;;) {
+        // primary march until a full batch is queued or every ray has left its loop
+        while (qn < 64 && __any(m.active)) {
+            V3 jp;
+            float js = 0.f, jx0 = 0.f, jw = 0.f;
+            bool job = false;
+            if (m.active) job = march_trip<FSET, COUNT>(c, tx, r, m, jp, js, jx0, jw, nd, nh);
+            const unsigned long long mask = __ballot(job);
+            if (job) {
+                const int slot = qn + __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                q.x[slot] = jp.x; q.y[slot] = jp.y; q.z[slot] = jp.z; q.step[slot] = js;
+                q.x0[slot] = jx0; q.w[slot] = jw; q.blend[slot] = r.beerBlend; q.owner[slot] = lane;
+            }
+            qn += __popcll(mask);
+        }
+        if (qn == 0) break;
+        __syncthreads();
+        const int nb = qn < 64 ? qn : 64;
+        if (lane < nb) {
+            float dal = cone_dal<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], nd, nh);
+            q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
+        }
+        __syncthreads();
+        for (int s = 0; s < nb; s++)
+            if (q.owner[s] == lane) transmittance = mixf(transmittance, q.res[s], q.w[s]);
+        const int rem = qn - nb;  // < 64 and only when nb == 64: source and destination slots are disjoint
+        __syncthreads();
+        if (lane < rem) {
+            const int s = nb + lane;
+            q.x[lane] = q.x[s]; q.y[lane] = q.y[s]; q.z[lane] = q.z[s]; q.step[lane] = q.step[s];
+            q.x0[lane] = q.x0[s]; q.w[lane] = q.w[s]; q.blend[lane] = q.blend[s]; q.owner[lane] = q.owner[s];
+        }
+        __syncthreads();
+        qn = rem;
+    }
+    if (valid && r.march) reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
     if (COUNT) {
         atomicAdd(&counts[0], (unsigned long long)nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
@@ -640,9 +813,11 @@ __global__ __launch_bounds__(256) void march_kernel(const MarchConsts c, const M
 
 template <int FSET, bool COUNT>
 int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts, hipStream_t s) {
-    const int tilesY = (c.gridH + 15) / 16;
-    dim3 grid((unsigned)(c.tilesX * tilesY)), block(256);
-    hipLaunchKernelGGL((march_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
+    dim3 grid((unsigned)(8 * c.tilesPerXcd)), block(64);
+    if (c.flags & MM_MARCH_NO_BATCH)
+        hipLaunchKernelGGL((march_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
+    else
+        hipLaunchKernelGGL((march_batched_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 

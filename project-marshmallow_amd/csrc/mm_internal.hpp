@@ -34,7 +34,8 @@ struct MarchConsts {
     int width, height, pitch;    // image size used for uv and bounds, row pitch in pixels
     int phase;                   // -1 = all pixels
     int gridW, gridH;            // marched pixel grid
-    int tilesX;
+    int tilesX, tilesY;           // 8x8 tiles of the marched grid
+    int tilesPerXcd;              // ceil(tilesX * tilesY / 8)
 };
 
 // Device view of one texture.

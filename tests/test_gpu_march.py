@@ -228,3 +228,23 @@ def test_compute_shader_operator_ping_pong(mm, gpu, tex):
     bi = b.cpu().numpy().reshape(H, W, 4)
     assert (ai[0::4, 0::4, 3] >= 0).all() and ai[1::4, 1::4].sum() == 0  # phase 0 only
     assert bi[1::4, 1::4, 3].max() > 0 and bi[0::4, 0::4].sum() == 0      # phase 5 only
+
+
+@pytest.mark.parametrize("fset,preset", [("fast", "scenario"), ("ref", "reference"), ("prec", "scenario")])
+def test_cone_batching_is_bit_identical(mm, gpu, orc, tex, fset, preset):
+    """The cone-batched kernel hands light-cone work to a wave-local queue and folds the results
+    back in creation order: it must reproduce the in-place (lock-step) kernel bit for bit."""
+    W, H = 200, 120  # ragged tiles
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H, sun_elevation_deg=15.0, time=150.0))
+    prm = mm.march_params(preset)
+    if fset == "prec":
+        ns = mm.NoiseSet("prec")
+        ns.generate(mm.noise_params(seed=3, weather_seed=2, coverage=0.6), mm.current_stream())
+    else:
+        ns = upload(mm, tex, fset)
+    a, ca = gpu_march(mm, ns, cam, sun, sky, W, H, params=prm, counts=True)
+    prm_lock = mm.march_params(preset)
+    prm_lock.flags |= mm.MARCH_NO_BATCH
+    b, cb = gpu_march(mm, ns, cam, sun, sky, W, H, params=prm_lock, counts=True)
+    assert np.array_equal(a, b)
+    assert (ca == cb).all()

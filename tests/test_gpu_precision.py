@@ -77,7 +77,7 @@ def test_precision_variant_error_vs_fp64(mm, gpu, orc):
     tex = orc.textures(t["weather"].reshape(512, 512, 4), t["curl"].reshape(128, 128, 4),
                        t["low_res"].reshape(128, 128, 128, 4), t["hi_res"].reshape(32, 32, 32, 4))
     r64 = orc.march_pixels(cam, sun, sky, tex, prm, W, H, xy, f64=True)
-    for fset, g in res.items():
+    for fset, g in list(res.items()):
         st = parity_stats(g, r64)
         print(f"{fset} vs FP64: {st}", flush=True)
         res[fset + "_stats"] = st
