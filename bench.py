@@ -133,10 +133,10 @@ def main():
 
     # instrumented run (outside the timed region): sample counts of this frame
     ns.generate(nparams, stream)
-    counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+    counts = torch.zeros(4, dtype=torch.int64, device="cuda")
     mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream, counts=counts)
     torch.cuda.synchronize()
-    n_density, n_detail = (int(v) for v in counts.cpu())
+    n_density, n_detail, n_jobs, _ = (int(v) for v in counts.cpu())
 
     for _ in range(args.warmup):
         ns.generate(nparams, stream)

@@ -185,8 +185,9 @@ typedef struct mm_march_params {
     int32_t subtexel_bits;      /* sampler weight precision: 8 = fixed-point like the hardware, 0 = float */
     /* light cone */
     int32_t cone_mode;          /* MM_CONE_* */
-    float surrogate_threshold;  /* predicted-error bound above which the plain cone is used */
-    const void* surrogate;      /* device pointer to packed surrogate weights (cone_mode 1) */
+    float surrogate_threshold;  /* jobs whose predicted error bound is >= this take the plain cone */
+    const void* surrogate;      /* device model block of mm_surrogate_bytes() (MM_CONE_SURROGATE);
+                                   REF and FAST texture sets only */
     int32_t _reserved[8];
 } mm_march_params;
 
@@ -215,13 +216,59 @@ int mm_march_clouds(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm
                     float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
                     int32_t phase, const mm_march_params* params, void* stream);
 
-/* Per-ray counters of the last launch of the instrumented march (density samples = cloudTest
- * evaluations, detail samples = cloudHiRes evaluations). counts: device int64[2] accumulators. */
+/* The march with work counters, accumulated (added) into counts: device uint64[4] =
+ * {density samples (cloudTest evaluations), detail samples (cloudHiRes evaluations),
+ *  light-cone jobs (fine-mode hits that light a sample, comp:440-464),
+ *  light-cone jobs evaluated by the plain six-sample cone (all of them unless MM_CONE_SURROGATE)}. */
 int mm_march_clouds_counted(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
                             const mm_noise_set* textures, const mm_texture* night_sky,
                             float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
                             int32_t phase, const mm_march_params* params,
                             unsigned long long* counts, void* stream);
+
+/* ---------------------------------------------------------------- learned light-cone surrogate
+ * No counterpart in the reference: an optional replacement of the six-sample light cone
+ * (comp:440-453) by a 16-64-64-64-2 SiLU MLP evaluated with BF16 MFMA inside the march. For each
+ * cone job it predicts the density along the light D and an error bound e (transmittance units);
+ * jobs with e >= params.surrogate_threshold take the plain cone, so threshold 0 is the plain march.
+ * A model is one device block of mm_surrogate_bytes() bytes (FP32 parameters, Adam state, feature
+ * statistics, packed BF16 fragments). Training samples are rows of MM_SURROGATE_ROW floats:
+ * 16 features followed by the cone's D, recorded by mm_march_clouds_collect. */
+#define MM_SURROGATE_FEATURES 16
+#define MM_SURROGATE_ROW 17
+#define MM_SURROGATE_PARAMS 9538   /* W1[16][64], b1[64], W2[64][64], b2, W3[64][64], b3, W4[64][2], b4[2] */
+
+size_t mm_surrogate_bytes(void);
+/* Random (seeded) weights, cleared optimiser state, identity feature normalisation. */
+int mm_surrogate_init(void* model, uint32_t seed, void* stream);
+/* Per-feature mean and inverse standard deviation over n sample rows (device). */
+int mm_surrogate_fit_stats(void* model, const float* samples, int32_t n, void* stream);
+/* `steps` Adam steps of `batch` rows (a multiple of 64) drawn from the n rows with a seeded hash.
+ * loss_out: NULL or device float[steps], the summed per-row loss of each step. Synchronises. */
+int mm_surrogate_train(void* model, const float* samples, int32_t n, int32_t batch, int32_t steps, float lr,
+                       uint32_t seed, float* loss_out, void* stream);
+/* Replace the FP32 parameters (device float[MM_SURROGATE_PARAMS]) and/or the feature statistics
+ * (device float[32]: mean[16], inverse std[16]) of a model, either may be NULL, and repack its BF16
+ * fragments: loading a saved model, or setting weights for a test. */
+int mm_surrogate_load(void* model, const float* params, const float* stats, void* stream);
+/* Exact mean gradient of the training loss over rows 0..n-1 (no sampling, no update): grad_out =
+ * device float[MM_SURROGATE_PARAMS] in the parameter layout, loss_out = device float, the summed
+ * per-row loss. For gradient checks against a host implementation. */
+int mm_surrogate_gradient(const void* model, const float* samples, int32_t n, float* grad_out, float* loss_out,
+                          void* stream);
+/* Batch inference over n rows of stride MM_SURROGATE_ROW: out[2i] = -D estimate, out[2i+1] = e. */
+int mm_surrogate_predict(const void* model, const float* samples, int32_t n, float* out, void* stream);
+
+/* The march with plain light cones that also records cone jobs as surrogate training rows: a job is
+ * kept when a hash of its position and `seed` is divisible by keep_one_in (1 = every job). Rows go
+ * to samples[capacity][MM_SURROGATE_ROW]; *count (device uint32, zeroed by the caller) receives the
+ * number of kept jobs, which may exceed capacity (extra rows are dropped). The image is identical
+ * to mm_march_clouds. REF and FAST texture sets. */
+int mm_march_clouds_collect(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                            const mm_noise_set* textures, const mm_texture* night_sky,
+                            float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
+                            int32_t phase, const mm_march_params* params, float* samples, uint32_t capacity,
+                            uint32_t keep_one_in, uint32_t seed, uint32_t* count, void* stream);
 
 /* ---------------------------------------------------------------- uniform builder (caller side) */
 /* Scene description from which the three UBOs are derived exactly as VulkanApplication.cpp:351-395

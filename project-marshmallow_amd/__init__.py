@@ -119,6 +119,21 @@ _SIGS = {
     "mm_march_clouds_counted": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
                                                ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                ctypes.c_int32, _P(MarchParams), ctypes.c_void_p, ctypes.c_void_p]),
+    "mm_surrogate_bytes": (ctypes.c_size_t, []),
+    "mm_surrogate_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "mm_surrogate_fit_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "mm_surrogate_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.c_float, ctypes.c_uint32, ctypes.c_void_p,
+                                          ctypes.c_void_p]),
+    "mm_surrogate_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "mm_surrogate_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p]),
+    "mm_surrogate_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                            ctypes.c_void_p]),
+    "mm_march_clouds_collect": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
+                                               ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32, _P(MarchParams), ctypes.c_void_p, ctypes.c_uint32,
+                                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),
     "mm_scene_default": (None, [_P(Scene)]),
     "mm_scene_scenario": (None, [_P(Scene)] + [ctypes.c_float] * 6),
     "mm_build_ubos": (ctypes.c_int, [_P(Scene), _P(CameraUBO), _P(SunUBO), _P(SkyUBO)]),
@@ -296,3 +311,82 @@ class ComputeShader:
         march_clouds(self.cam, self.sun, self.sky, self.noise, target, self.width, self.height,
                      phase=PHASE_FROM_UBO, params=self.params, night=self.night, stream=stream)
         return target
+
+
+# ---------------------------------------------------------------- learned light-cone surrogate
+SURROGATE_FEATURES = 16
+SURROGATE_ROW = 17
+# parameter layout of the FP32 master copy (offsets in floats): W1[16][64], b1, W2[64][64], b2,
+# W3[64][64], b3, W4[64][2], b4[2]
+SURROGATE_PARAMS = 9538
+
+
+class Surrogate:
+    """One device model block (parameters, Adam state, feature statistics, BF16 fragments)."""
+
+    def __init__(self, seed: int = 1, device="cuda", stream: int | None = None):
+        import torch
+
+        self.block = torch.zeros(int(lib().mm_surrogate_bytes()), dtype=torch.uint8, device=device)
+        stream = current_stream() if stream is None else stream
+        check(lib().mm_surrogate_init(self.ptr, seed, stream), "mm_surrogate_init")
+
+    @property
+    def ptr(self) -> int:
+        return self.block.data_ptr()
+
+    def params(self):
+        """View of the FP32 master parameters (float32 tensor of SURROGATE_PARAMS)."""
+        return self.block[: SURROGATE_PARAMS * 4].view(dtype=__import__("torch").float32)
+
+    def fit_stats(self, samples, n: int | None = None, stream: int | None = None) -> None:
+        n = samples.shape[0] if n is None else n
+        stream = current_stream() if stream is None else stream
+        check(lib().mm_surrogate_fit_stats(self.ptr, samples.data_ptr(), n, stream), "mm_surrogate_fit_stats")
+
+    def train(self, samples, steps: int, batch: int = 8192, lr: float = 1e-3, seed: int = 1, n: int | None = None,
+              loss_out=None, stream: int | None = None) -> None:
+        n = samples.shape[0] if n is None else n
+        stream = current_stream() if stream is None else stream
+        lp = loss_out.data_ptr() if loss_out is not None else None
+        check(lib().mm_surrogate_train(self.ptr, samples.data_ptr(), n, batch, steps, lr, seed, lp, stream),
+              "mm_surrogate_train")
+
+    def load(self, params=None, stats=None, stream: int | None = None) -> None:
+        """Set FP32 parameters (float32 device tensor [SURROGATE_PARAMS]) and/or stats ([32])."""
+        stream = current_stream() if stream is None else stream
+        pp = params.data_ptr() if params is not None else None
+        sp = stats.data_ptr() if stats is not None else None
+        check(lib().mm_surrogate_load(self.ptr, pp, sp, stream), "mm_surrogate_load")
+
+    def gradient(self, samples, grad_out, loss_out, n: int | None = None, stream: int | None = None) -> None:
+        n = samples.shape[0] if n is None else n
+        stream = current_stream() if stream is None else stream
+        check(lib().mm_surrogate_gradient(self.ptr, samples.data_ptr(), n, grad_out.data_ptr(), loss_out.data_ptr(),
+                                          stream), "mm_surrogate_gradient")
+
+    def stats(self):
+        """Feature mean[16] and inverse std[16] (float32 tensor view)."""
+        import torch
+
+        off = 4 * ((SURROGATE_PARAMS * 4 + 255) // 256 * 256)
+        return self.block[off: off + 32 * 4].view(dtype=torch.float32)
+
+    def predict(self, samples, out, n: int | None = None, stream: int | None = None) -> None:
+        n = samples.shape[0] if n is None else n
+        stream = current_stream() if stream is None else stream
+        check(lib().mm_surrogate_predict(self.ptr, samples.data_ptr(), n, out.data_ptr(), stream),
+              "mm_surrogate_predict")
+
+
+def march_collect(cam: CameraUBO, sun: SunUBO, sky: SkyUBO, noise: NoiseSet, out, width: int, height: int,
+                  samples, count, keep_one_in: int = 1, seed: int = 1, phase: int = PHASE_ALL,
+                  params: MarchParams | None = None, stream: int | None = None) -> None:
+    """Plain march that also records surrogate training rows into ``samples`` (float32 [cap, 17])."""
+    stream = current_stream() if stream is None else stream
+    pp = ctypes.byref(params) if params is not None else None
+    code = lib().mm_march_clouds_collect(ctypes.byref(cam), ctypes.byref(sun), ctypes.byref(sky),
+                                         ctypes.byref(noise.c), None, out.data_ptr(), width, height, width, phase, pp,
+                                         samples.data_ptr(), samples.shape[0], keep_one_in, seed, count.data_ptr(),
+                                         stream)
+    check(code, "mm_march_clouds_collect")

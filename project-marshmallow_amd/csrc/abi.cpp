@@ -88,6 +88,11 @@ mm::TexView view_of(const mm_texture& t) {
     return v;
 }
 
+// hipGetLastError() reports the last error of any earlier runtime call on this thread, including
+// benign ones made by the caller's own runtime set-up; launch status is read with it, so it is
+// cleared before this library's launches.
+void reset_error_state() { (void)hipGetLastError(); }
+
 int check_params(const mm_march_params* p) {
     if (p->size != sizeof(mm_march_params)) return MM_E_VERSION;
     const float f[] = {p->inner_diameter, p->outer_diameter, p->center_y, p->thickness, p->proj_radius,
@@ -98,7 +103,9 @@ int check_params(const mm_march_params* p) {
     if (!(p->coarse_step_frac > 0.f) || !(p->fine_step_mul > 0.f) || p->fine_step_mul > 1.f) return MM_E_PARAM;
     if (p->miss_limit < 1) return MM_E_PARAM;
     if (p->subtexel_bits != 0 && p->subtexel_bits != 8) return MM_E_PARAM;
-    if (p->cone_mode != MM_CONE_PLAIN) return MM_E_PARAM;
+    if (p->cone_mode != MM_CONE_PLAIN && p->cone_mode != MM_CONE_SURROGATE) return MM_E_PARAM;
+    if (p->cone_mode == MM_CONE_SURROGATE && (!p->surrogate || !(p->surrogate_threshold >= 0.f)))
+        return MM_E_PARAM;
     return MM_OK;
 }
 
@@ -210,6 +217,7 @@ int mm_generate_noise(const mm_noise_params* p, const mm_noise_set* out, void* w
             out->low_res.layout != MM_LAYOUT_LINEAR)
             return MM_E_FORMAT;
     }
+    reset_error_state();
     void* ws = workspace;
     hipStream_t s = (hipStream_t)stream;
     bool own = false;
@@ -231,13 +239,20 @@ int mm_pack_texture(int32_t kind, const mm_texture* src, const mm_texture* dst, 
     if (src->format != MM_FMT_RGBA8 || src->layout != MM_LAYOUT_LINEAR) return MM_E_FORMAT;
     if ((e = check_texture(dst, kind, dims, false))) return e;
     if (dst->width != src->width || dst->height != src->height || dst->depth != src->depth) return MM_E_DIM;
+    reset_error_state();
     return mm::launch_pack(kind, src, dst, stream);
 }
+
+struct CollectArgs {
+    float* samples;
+    uint32_t capacity, keep, seed;
+    uint32_t* count;
+};
 
 static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
                       const mm_noise_set* tex, const mm_texture* night_sky, float* out, int32_t width, int32_t height,
                       int32_t pitch, int32_t phase, const mm_march_params* params, unsigned long long* counts,
-                      void* stream) {
+                      const CollectArgs* collect, void* stream) {
     if (!camera || !sun || !sky || !tex || !out) return MM_E_NULL;
     if (width <= 0 || height <= 0 || width > 32768 || height > 32768) return MM_E_DIM;
     if (pitch < width) return MM_E_PITCH;
@@ -300,6 +315,9 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
     for (int i = 0; i < 6; i++)
         for (int k = 0; k < 3; k++)
             c.coneOffsets[i][k] = ((B[k] * cone[i][0]) + (B[4 + k] * cone[i][1])) + (B[8 + k] * cone[i][2]);
+    const float probe[3] = {0.6f, 2.5f, 6.0f};  // surrogate features: axial offsets of the cone table
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) c.probeOffsets[i][k] = B[4 + k] * probe[i];
     c.earthCenter[0] = c.camPos[0];
     c.earthCenter[1] = params->center_y;
     c.earthCenter[2] = c.camPos[2];
@@ -354,6 +372,23 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
         c.gridW = width;
         c.gridH = height;
     }
+    c.coneMode = params->cone_mode;
+    if (c.coneMode == MM_CONE_SURROGATE) {
+        if (fset == mm::FSET_PREC) return MM_E_FORMAT;
+        c.surrThreshold = params->surrogate_threshold;
+        c.surrParams = mm::surrogate_params(params->surrogate);
+        c.surrStats = mm::surrogate_stats(params->surrogate);
+        c.surrFrags = mm::surrogate_frags(params->surrogate);
+    }
+    if (collect) {
+        if (fset == mm::FSET_PREC) return MM_E_FORMAT;
+        c.coneMode = 2;  // plain cones + recorder
+        c.collectBuf = collect->samples;
+        c.collectCap = collect->capacity;
+        c.collectKeep = collect->keep;
+        c.collectSeed = collect->seed;
+        c.collectCount = collect->count;
+    }
     if (c.gridW <= 0 || c.gridH <= 0) return MM_OK;  // no pixel of this phase in the image
     c.tilesX = (c.gridW + 7) / 8;
     c.tilesY = (c.gridH + 7) / 8;
@@ -369,6 +404,7 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
         t.night = view_of(*night_sky);
         t.hasNight = 1;
     }
+    reset_error_state();
     return mm::launch_march(fset, c, t, out, counts, stream);
 }
 
@@ -376,7 +412,7 @@ int mm_march_clouds(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm
                     const mm_noise_set* textures, const mm_texture* night_sky, float* out_rgba32f, int32_t width,
                     int32_t height, int32_t row_pitch_px, int32_t phase, const mm_march_params* params, void* stream) {
     return march_impl(camera, sun, sky, textures, night_sky, out_rgba32f, width, height, row_pitch_px, phase, params,
-                      nullptr, stream);
+                      nullptr, nullptr, stream);
 }
 
 int mm_march_clouds_counted(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
@@ -385,7 +421,66 @@ int mm_march_clouds_counted(const mm_camera_ubo* camera, const mm_sun_ubo* sun, 
                             const mm_march_params* params, unsigned long long* counts, void* stream) {
     if (!counts) return MM_E_NULL;
     return march_impl(camera, sun, sky, textures, night_sky, out_rgba32f, width, height, row_pitch_px, phase, params,
-                      counts, stream);
+                      counts, nullptr, stream);
+}
+
+int mm_march_clouds_collect(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                            const mm_noise_set* textures, const mm_texture* night_sky, float* out_rgba32f,
+                            int32_t width, int32_t height, int32_t row_pitch_px, int32_t phase,
+                            const mm_march_params* params, float* samples, uint32_t capacity, uint32_t keep_one_in,
+                            uint32_t seed, uint32_t* count, void* stream) {
+    if (!samples || !count) return MM_E_NULL;
+    if (capacity == 0 || keep_one_in == 0) return MM_E_PARAM;
+    if (params && params->cone_mode != MM_CONE_PLAIN) return MM_E_PARAM;
+    CollectArgs a{samples, capacity, keep_one_in, seed, count};
+    return march_impl(camera, sun, sky, textures, night_sky, out_rgba32f, width, height, row_pitch_px, phase, params,
+                      nullptr, &a, stream);
+}
+
+size_t mm_surrogate_bytes(void) { return mm::surrogate_model_bytes(); }
+
+int mm_surrogate_init(void* model, uint32_t seed, void* stream) {
+    if (!model) return MM_E_NULL;
+    reset_error_state();
+    return mm::surrogate_init(model, seed, stream);
+}
+
+int mm_surrogate_fit_stats(void* model, const float* samples, int32_t n, void* stream) {
+    if (!model || !samples) return MM_E_NULL;
+    if (n < 2) return MM_E_PARAM;
+    reset_error_state();
+    return mm::surrogate_fit_stats(model, samples, n, MM_SURROGATE_ROW, stream);
+}
+
+int mm_surrogate_train(void* model, const float* samples, int32_t n, int32_t batch, int32_t steps, float lr,
+                       uint32_t seed, float* loss_out, void* stream) {
+    if (!model || !samples) return MM_E_NULL;
+    if (n < 1 || steps < 0 || batch < 64 || batch % 64 != 0 || batch > (1 << 24) || !(lr > 0.f) || !isfinite(lr))
+        return MM_E_PARAM;
+    reset_error_state();
+    return mm::surrogate_train(model, samples, n, MM_SURROGATE_ROW, batch, steps, lr, seed, loss_out, stream);
+}
+
+int mm_surrogate_load(void* model, const float* params, const float* stats, void* stream) {
+    if (!model) return MM_E_NULL;
+    reset_error_state();
+    return mm::surrogate_load(model, params, stats, stream);
+}
+
+int mm_surrogate_gradient(const void* model, const float* samples, int32_t n, float* grad_out, float* loss_out,
+                          void* stream) {
+    if (!model || !samples || !grad_out || !loss_out) return MM_E_NULL;
+    if (n < 1 || n > (1 << 26)) return MM_E_PARAM;
+    reset_error_state();
+    return mm::surrogate_gradient(model, samples, n, MM_SURROGATE_ROW, grad_out, loss_out, stream);
+}
+
+int mm_surrogate_predict(const void* model, const float* samples, int32_t n, float* out, void* stream) {
+    if (!model || !samples || !out) return MM_E_NULL;
+    if (n < 0) return MM_E_PARAM;
+    if (n == 0) return MM_OK;
+    reset_error_state();
+    return mm::surrogate_predict(model, samples, n, MM_SURROGATE_ROW, out, stream);
 }
 
 }  // extern "C"

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include "mm_internal.hpp"
+#include "surrogate_dev.hpp"
 
 namespace {
 
@@ -585,6 +586,34 @@ __device__ __forceinline__ float beers_of(float dal, float beerBlend) {
     return mixf(beers, beersMod, beerBlend);
 }
 
+// Inputs of the light-cone surrogate for one job: hit state (relative height, low-res density,
+// eroded density), weather coverage/type at the hit, sun elevation, three density probes along the
+// sun direction (the cone's axial offsets 0.6, 2.5 and 6 step lengths x 3), step scale, and the
+// position within the hi-res tile as sin/cos pairs.
+template <int FSET, bool COUNT>
+__device__ __forceinline__ void cone_features(const MarchConsts& c, const MarchTextures& tx, V3 pos, float stepSize,
+                                              float rh, float lo, float dens, float* f, int& nd) {
+    V3 center = ld3(c.camPos);
+    center.y = c.earthCenter[1];
+    const V3 p = pos + wind_offset(c, rh);
+    const V3 proj = proj_shell(c, p, center);
+    const Weather w = read_weather<FSET>(tx.weather, c.weatherScale * (proj.x - c.camPos[0]),
+                                         c.weatherScale * (proj.z - c.camPos[2]), c.subtexelBits);
+    f[0] = rh; f[1] = lo; f[2] = dens; f[3] = w.coverage; f[4] = w.type; f[5] = c.sunDir[1];
+#pragma unroll 1
+    for (int i = 0; i < 3; i++) {
+        V3 lsPos = pos + ((3.0f * stepSize) * ld3(c.probeOffsets[i]));
+        V3 lsProj = proj_shell(c, lsPos, center);
+        float lsHeight = rel_height(c, lsPos, lsProj);
+        f[6 + i] = cloud_test<FSET, COUNT>(c, tx, lsPos + wind_offset(c, lsHeight), lsHeight, center, nd);
+    }
+    f[9] = stepSize * c.invThickness;
+    const float k = 6.283185307179586f * c.hiScale;
+    f[10] = __sinf(k * p.x); f[11] = __cosf(k * p.x);
+    f[12] = __sinf(k * p.y); f[13] = __cosf(k * p.y);
+    f[14] = __sinf(k * p.z); f[15] = __cosf(k * p.z);
+}
+
 // in-scatter x phase of a fine hit (comp:461-462, 464): the factor that multiplies Beer's law
 __device__ __forceinline__ float inscatter_hg(float loDensity, float rHeight, float HG) {
     float e1 = clampf(remapk(rHeight, 0.3f, 1.0f / (0.85f - 0.3f), 0.5f, 2.0f - 0.5f), 0.5f, 2.0f);
@@ -617,7 +646,7 @@ __device__ __forceinline__ void march_init(const MarchConsts& c, const RaySetup&
 template <int FSET, bool COUNT>
 __device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchTextures& tx, const RaySetup& r,
                                            MarchState& m, V3& jobPos, float& jobStep, float& jobX0, float& jobW,
-                                           int& nd, int& nh) {
+                                           float& jobRh, float& jobLo, float& jobDens, int& nd, int& nh) {
     bool job = false;
     // hard bound on loop trips: guarantees termination even if t += stepSize stops advancing t
     if (++m.iterations > MM_MAX_MARCH_ITERATIONS) { m.active = false; return false; }
@@ -654,6 +683,9 @@ __device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchText
                 jobStep = m.stepSize;
                 jobX0 = inscatter_hg(loDensity, rHeight, r.HG);
                 jobW = 1.0f - m.accum;
+                jobRh = rHeight;
+                jobLo = loDensity;
+                jobDens = density;
                 m.accum += density;
             }
         }
@@ -704,7 +736,7 @@ __global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const Ma
     if (gi >= c.gridW || gj >= c.gridH) return;
     int px, py;
     tile_pixel(c, gi, gj, px, py);
-    int nd = 0, nh = 0;
+    int nd = 0, nh = 0, nj = 0, nc = 0;  // density, detail samples; cone jobs, jobs on the plain cone
     const RaySetup r = ray_setup(c, tx, px, py);
     float4* dst = reinterpret_cast<float4*>(out) + ((size_t)py * c.pitch + px);
     if (!r.march) {
@@ -716,16 +748,19 @@ __global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const Ma
     float transmittance = 1.0f;
     while (m.active) {
         V3 jp;
-        float js, jx0, jw;
-        if (march_trip<FSET, COUNT>(c, tx, r, m, jp, js, jx0, jw, nd, nh)) {
+        float js, jx0, jw, jrh, jlo, jd;
+        if (march_trip<FSET, COUNT>(c, tx, r, m, jp, js, jx0, jw, jrh, jlo, jd, nd, nh)) {
             float dal = cone_dal<FSET, COUNT>(c, tx, jp, js, nd, nh);
             transmittance = mixf(transmittance, jx0 * beers_of(dal, r.beerBlend), jw);
+            if (COUNT) { nj++; nc++; }
         }
     }
     *dst = compose(c, r, m.accum, transmittance);
     if (COUNT) {
         atomicAdd(&counts[0], (unsigned long long)nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
+        atomicAdd(&counts[2], (unsigned long long)nj);
+        atomicAdd(&counts[3], (unsigned long long)nc);
     }
 }
 
@@ -741,13 +776,23 @@ constexpr int kQueueCap = 128;
 struct ConeQueue {
     float x[kQueueCap], y[kQueueCap], z[kQueueCap], step[kQueueCap];
     float x0[kQueueCap], w[kQueueCap], blend[kQueueCap], res[kQueueCap];
+    float rh[kQueueCap], lo[kQueueCap], dens[kQueueCap];
     int owner[kQueueCap];
+    int fallback[64];
+    float dal[64];
 };
 
-template <int FSET, bool COUNT>
-__global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, const MarchTextures tx,
-                                                           float* __restrict__ out,
-                                                           unsigned long long* __restrict__ counts) {
+// light-cone modes: MM_CONE_PLAIN, MM_CONE_SURROGATE, and the training-sample recorder
+enum ConeMode { CONE_PLAIN = MM_CONE_PLAIN, CONE_SURROGATE = MM_CONE_SURROGATE, CONE_COLLECT = 2 };
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+
+template <int FSET, bool COUNT, int MODE>
+__device__ __forceinline__ void march_batched_body(const MarchConsts& c, const MarchTextures& tx,
+                                                   float* __restrict__ out, unsigned long long* __restrict__ counts) {
     __shared__ ConeQueue q;
     const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
     if (tile >= c.tilesX * c.tilesY) return;  // uniform across the block
@@ -756,7 +801,7 @@ __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, 
     const bool valid = gi < c.gridW && gj < c.gridH;
     int px = 0, py = 0;
     if (valid) tile_pixel(c, gi, gj, px, py);
-    int nd = 0, nh = 0;
+    int nd = 0, nh = 0, nj = 0, nc = 0;  // density, detail samples; cone jobs, jobs on the plain cone
     RaySetup r;
     MarchState m;
     m.active = false;
@@ -772,24 +817,82 @@ __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, 
         // primary march until a full batch is queued or every ray has left its loop
         while (qn < 64 && __any(m.active)) {
             V3 jp;
-            float js = 0.f, jx0 = 0.f, jw = 0.f;
+            float js = 0.f, jx0 = 0.f, jw = 0.f, jrh = 0.f, jlo = 0.f, jd = 0.f;
             bool job = false;
-            if (m.active) job = march_trip<FSET, COUNT>(c, tx, r, m, jp, js, jx0, jw, nd, nh);
+            if (m.active) job = march_trip<FSET, COUNT>(c, tx, r, m, jp, js, jx0, jw, jrh, jlo, jd, nd, nh);
             const unsigned long long mask = __ballot(job);
             if (job) {
                 const int slot = qn + __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
                 q.x[slot] = jp.x; q.y[slot] = jp.y; q.z[slot] = jp.z; q.step[slot] = js;
                 q.x0[slot] = jx0; q.w[slot] = jw; q.blend[slot] = r.beerBlend; q.owner[slot] = lane;
+                q.rh[slot] = jrh; q.lo[slot] = jlo; q.dens[slot] = jd;
             }
             qn += __popcll(mask);
         }
         if (qn == 0) break;
         __syncthreads();
         const int nb = qn < 64 ? qn : 64;
-        if (lane < nb) {
-            float dal = cone_dal<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], nd, nh);
-            q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
+        if (COUNT) nj += nb;
+        if constexpr (MODE == CONE_PLAIN) {
+            if (COUNT) nc += nb;
+            if (lane < nb) {
+                float dal = cone_dal<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], nd, nh);
+                q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
+            }
+        } else if constexpr (MODE == CONE_COLLECT) {
+            if (COUNT) nc += nb;
+            // plain cone + record (features, D) of one job in c.collectKeep
+            float f[surr::kIn];
+            float dal = 0.0f;
+            bool keep = false;
+            if (lane < nb) {
+                const V3 pos = mk(q.x[lane], q.y[lane], q.z[lane]);
+                dal = cone_dal<FSET, COUNT>(c, tx, pos, q.step[lane], nd, nh);
+                q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
+                const uint32_t hsh = mix32(__float_as_uint(pos.x) ^ mix32(__float_as_uint(pos.y) ^
+                                           mix32(__float_as_uint(pos.z) ^ c.collectSeed)));
+                keep = (hsh % c.collectKeep) == 0u;
+                if (keep) cone_features<FSET, COUNT>(c, tx, pos, q.step[lane], q.rh[lane], q.lo[lane], q.dens[lane], f, nd);
+            }
+            const unsigned long long km = __ballot(keep);
+            if (km) {
+                unsigned base = 0;
+                if (lane == 0) base = atomicAdd(c.collectCount, (unsigned)__popcll(km));
+                base = __shfl(base, 0);
+                const unsigned slot = base + __builtin_amdgcn_mbcnt_hi((unsigned)(km >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
+                if (keep && slot < c.collectCap) {
+                    float* dst = c.collectBuf + (size_t)slot * (surr::kIn + 1);
+                    for (int k = 0; k < surr::kIn; k++) dst[k] = f[k];
+                    dst[surr::kIn] = dal;
+                }
+            }
+        } else {
+            // surrogate: features + MLP for every queued job, plain cone only where the predicted
+            // error bound is not below the threshold (compacted onto the first lanes)
+            float f[surr::kIn];
+            for (int k = 0; k < surr::kIn; k++) f[k] = 0.0f;
+            if (lane < nb)
+                cone_features<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], q.rh[lane],
+                                           q.lo[lane], q.dens[lane], f, nd);
+            for (int k = 0; k < surr::kIn; k++) f[k] = (f[k] - c.surrStats[k]) * c.surrStats[16 + k];
+            const surr::FwdOut o = surr::mlp_forward64(c.surrFrags, c.surrParams, f);
+            const bool fb = lane < nb && !(surr::softplus(o.o1) < c.surrThreshold);
+            const unsigned long long fm = __ballot(fb);
+            const int nf = __popcll(fm);
+            if (fb) q.fallback[__builtin_amdgcn_mbcnt_hi((unsigned)(fm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)fm, 0u))] = lane;
+            __syncthreads();
+            if (COUNT) nc += nf;
+            if (lane < nf) {
+                const int j = q.fallback[lane];
+                q.dal[j] = cone_dal<FSET, COUNT>(c, tx, mk(q.x[j], q.y[j], q.z[j]), q.step[j], nd, nh);
+            }
+            __syncthreads();
+            if (lane < nb) {
+                const float dal = fb ? q.dal[lane] : fmaxf(0.0f, -o.o0);
+                q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
+            }
         }
         __syncthreads();
         for (int s = 0; s < nb; s++)
@@ -800,6 +903,7 @@ __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, 
             const int s = nb + lane;
             q.x[lane] = q.x[s]; q.y[lane] = q.y[s]; q.z[lane] = q.z[s]; q.step[lane] = q.step[s];
             q.x0[lane] = q.x0[s]; q.w[lane] = q.w[s]; q.blend[lane] = q.blend[s]; q.owner[lane] = q.owner[s];
+            q.rh[lane] = q.rh[s]; q.lo[lane] = q.lo[s]; q.dens[lane] = q.dens[s];
         }
         __syncthreads();
         qn = rem;
@@ -808,16 +912,47 @@ __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, 
     if (COUNT) {
         atomicAdd(&counts[0], (unsigned long long)nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
+        if (lane == 0) {
+            atomicAdd(&counts[2], (unsigned long long)nj);
+            atomicAdd(&counts[3], (unsigned long long)nc);
+        }
     }
+}
+
+template <int FSET, bool COUNT, int MODE>
+__global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, const MarchTextures tx,
+                                                           float* __restrict__ out,
+                                                           unsigned long long* __restrict__ counts) {
+    march_batched_body<FSET, COUNT, MODE>(c, tx, out, counts);
+}
+
+#ifndef MM_SURROGATE_WAVES_PER_EU
+#define MM_SURROGATE_WAVES_PER_EU 4
+#endif
+// The surrogate variant bounds its register allocation: the MLP's peak (accumulators, operands)
+// otherwise sets the occupancy of the whole march.
+template <int FSET, bool COUNT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
+void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
+                            unsigned long long* __restrict__ counts) {
+    march_batched_body<FSET, COUNT, CONE_SURROGATE>(c, tx, out, counts);
 }
 
 template <int FSET, bool COUNT>
 int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts, hipStream_t s) {
     dim3 grid((unsigned)(8 * c.tilesPerXcd)), block(64);
-    if (c.flags & MM_MARCH_NO_BATCH)
+    if (c.coneMode == CONE_PLAIN && (c.flags & MM_MARCH_NO_BATCH))
         hipLaunchKernelGGL((march_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
-    else
-        hipLaunchKernelGGL((march_batched_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
+    else if (c.coneMode == CONE_PLAIN)
+        hipLaunchKernelGGL((march_batched_kernel<FSET, COUNT, CONE_PLAIN>), grid, block, 0, s, c, t, out, counts);
+    else if constexpr (FSET != mm::FSET_PREC) {
+        if (c.coneMode == CONE_COLLECT)
+            hipLaunchKernelGGL((march_batched_kernel<FSET, COUNT, CONE_COLLECT>), grid, block, 0, s, c, t, out, counts);
+        else
+            hipLaunchKernelGGL((march_surrogate_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
+    } else {
+        return MM_E_FORMAT;
+    }
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
+#include <hip/hip_runtime.h>
 #include "../../include/mm.h"
 
 namespace mm {
@@ -36,6 +37,16 @@ struct MarchConsts {
     int gridW, gridH;            // marched pixel grid
     int tilesX, tilesY;           // 8x8 tiles of the marched grid
     int tilesPerXcd;              // ceil(tilesX * tilesY / 8)
+    float probeOffsets[3][3];     // basis * (0, k, 0), k = 0.6, 2.5, 6: surrogate density probes
+    // light-cone mode: MM_CONE_PLAIN, MM_CONE_SURROGATE, or 2 = plain + record surrogate training samples
+    int coneMode;
+    float surrThreshold;
+    const float* surrParams;      // FP32 parameters of the surrogate MLP
+    const float* surrStats;       // feature mean[16], inverse std[16]
+    const uint4* surrFrags;       // BF16 MFMA weight fragments
+    float* collectBuf;            // [collectCap][17]: 16 features + density along the light
+    unsigned* collectCount;
+    unsigned collectCap, collectKeep, collectSeed;
 };
 
 // Device view of one texture.
@@ -53,6 +64,19 @@ struct MarchTextures {
     TexView weather, curl, low, hi, night;
     int hasNight;
 };
+
+size_t surrogate_model_bytes();
+int surrogate_init(void* model, uint32_t seed, void* stream);
+int surrogate_fit_stats(void* model, const float* samples, int n, int stride, void* stream);
+int surrogate_train(void* model, const float* samples, int n, int stride, int batch, int steps, float lr,
+                    uint32_t seed, float* loss_out, void* stream);
+int surrogate_predict(const void* model, const float* x, int n, int stride, float* out, void* stream);
+int surrogate_load(void* model, const float* params, const float* stats, void* stream);
+int surrogate_gradient(const void* model, const float* samples, int n, int stride, float* grad_out, float* loss_out,
+                       void* stream);
+const float* surrogate_params(const void* model);
+const float* surrogate_stats(const void* model);
+const uint4* surrogate_frags(const void* model);
 
 int launch_march(int fset, const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts,
                  void* stream);

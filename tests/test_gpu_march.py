@@ -43,7 +43,7 @@ def gpu_march(mm, ns, cam, sun, sky, W, H, phase=-1, params=None, night=None, fi
     import torch
 
     out = torch.full((H * W * 4,), fill, dtype=torch.float32, device="cuda")
-    cnt = torch.zeros(2, dtype=torch.int64, device="cuda") if counts else None
+    cnt = torch.zeros(4, dtype=torch.int64, device="cuda") if counts else None
     mm.march_clouds(cam, sun, sky, ns, out, W, H, phase=phase, params=params, night=night, counts=cnt)
     torch.cuda.synchronize()
     img = out.cpu().numpy().reshape(H, W, 4)
@@ -177,7 +177,9 @@ def test_1080p_properties_and_sampled_parity(mm, gpu, orc, tex):
     a = gpu_march(mm, ns, cam, sun, sky, W, H, params=prm)
     b = gpu_march(mm, ns, cam, sun, sky, W, H, params=prm)
     assert np.array_equal(a, b)  # deterministic
-    assert np.isfinite(a).all() and (a[..., 3] >= 0).all() and (a[..., 3] <= 1).all()
+    # alpha = max(skyAmbient, sunDisk) * (1 - accum) (comp:346, 496): up to 1 + O(1e-4) where
+    # dot(sunDir, rayDirection) rounds above 1 and is raised to the 768th power (comp:330-334)
+    assert np.isfinite(a).all() and (a[..., 3] >= 0).all() and (a[..., 3] <= 1.001).all()
     rng = np.random.default_rng(1)
     xy = np.stack([rng.integers(0, W, 4096), rng.integers(0, H, 4096)], 1).astype(np.int32)
     r = orc.march_pixels(cam, sun, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]), prm, W, H,

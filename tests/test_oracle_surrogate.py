@@ -1,0 +1,86 @@
+"""Pins the surrogate oracle (CPU): bfloat16 rounding against torch's conversion and exact ties,
+the float64 gradient against central finite differences of the float64 loss."""
+import numpy as np
+import pytest
+
+import surrogate_mlp as S
+
+
+def _rows(rng, n):
+    x = rng.normal(0.0, 1.0, (n, 17))
+    x[:, 16] = rng.uniform(0.0, 4.0, n)  # density along the light
+    return x.astype(np.float32)
+
+
+def test_bf16_rounding_ties_and_torch():
+    torch = pytest.importorskip("torch")
+    one = 1.0
+    assert S.bf16(np.array([one]))[0] == 1.0
+    assert S.bf16(np.array([1.0 + 2.0 ** -8]))[0] == 1.0                    # tie -> even (mantissa 0)
+    assert S.bf16(np.array([1.0 + 3 * 2.0 ** -8]))[0] == 1.0 + 2.0 ** -6    # tie -> even (mantissa 2)
+    assert S.bf16(np.array([-(1.0 + 3 * 2.0 ** -8)]))[0] == -(1.0 + 2.0 ** -6)
+    rng = np.random.default_rng(3)
+    v = np.concatenate([rng.normal(0, 10, 5000), rng.normal(0, 1e-3, 5000)]).astype(np.float32)
+    ref = torch.from_numpy(v).to(torch.bfloat16).to(torch.float32).numpy().astype(np.float64)
+    assert np.array_equal(S.bf16(v), ref)
+
+
+def test_param_layout():
+    assert S.N_PARAMS == 9538
+    flat = np.arange(S.N_PARAMS, dtype=np.float64)
+    p = S.unpack(flat)
+    assert p["W1"].shape == (16, 64) and p["W1"][1, 0] == 64  # W[k_in][unit] row-major
+    assert p["b4"].shape == (2,) and p["b4"][1] == S.N_PARAMS - 1
+    assert np.array_equal(S.pack(p), flat)
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(7)
+    flat = S.init_params(5)
+    stats = np.concatenate([rng.normal(0, 0.3, 16), rng.uniform(0.5, 2.0, 16)]).astype(np.float32)
+    rows = _rows(rng, 6)
+    g, L = S.gradient(flat, stats, rows)
+    assert np.isclose(L, S.loss(flat, stats, rows))
+    # |r| is held fixed by the loss's second term: differentiate with it frozen
+    o, _ = S.forward(flat, S.normalise(rows, stats))
+    abs_r = np.abs(np.exp(o[:, 0]) - np.exp(-rows[:, 16].astype(np.float64)))
+    n = len(rows)
+    # every bias and a sample of weights from each layer
+    off = {}
+    o_ = 0
+    for name, shape in S.SHAPES:
+        off[name] = (o_, int(np.prod(shape)))
+        o_ += int(np.prod(shape))
+    idx = []
+    for name, (a, m) in off.items():
+        idx += list(a + rng.choice(m, size=min(m, 12), replace=False))
+    h = 1e-6
+    for i in idx:
+        fp, fm = flat.copy(), flat.copy()
+        fp[i] += h
+        fm[i] -= h
+        fd = (S.loss_fixed_abs(fp, stats, rows, abs_r) - S.loss_fixed_abs(fm, stats, rows, abs_r)) / (2 * h) / n
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_planted_transpose_is_caught():
+    """A transposed hidden weight (a plausible indexing slip) changes the gradient visibly."""
+    rng = np.random.default_rng(2)
+    flat = S.init_params(9)
+    stats = np.concatenate([np.zeros(16), np.ones(16)]).astype(np.float32)
+    rows = _rows(rng, 64)
+    p = S.unpack(flat)
+    p["W2"] = p["W2"].T.copy()
+    g0, _ = S.gradient(flat, stats, rows)
+    g1, _ = S.gradient(S.pack(p), stats, rows)
+    assert np.max(np.abs(g0 - g1)) > 1e-3 * np.max(np.abs(g0))
+
+
+def test_bf16_gradient_close_to_float64():
+    rng = np.random.default_rng(11)
+    flat = S.init_params(3)
+    stats = np.concatenate([np.zeros(16), np.ones(16)]).astype(np.float32)
+    rows = _rows(rng, 256)
+    g64, _ = S.gradient(flat, stats, rows)
+    g16, _ = S.gradient(flat, stats, rows, bf16_ops=True)
+    assert np.max(np.abs(g64 - g16)) <= 0.05 * np.max(np.abs(g64))
