@@ -188,7 +188,9 @@ typedef struct mm_march_params {
     float surrogate_threshold;  /* jobs whose predicted error bound is >= this take the plain cone */
     const void* surrogate;      /* device model block of mm_surrogate_bytes() (MM_CONE_SURROGATE);
                                    REF and FAST texture sets only */
-    int32_t _reserved[8];
+    int32_t tile_width;         /* pixels per wave tile row: 8 (8x8, default when 0), 16 (16x4), 32 (32x2);
+                                   the image does not depend on it */
+    int32_t _reserved[7];
 } mm_march_params;
 
 /* Preset with the reference's constants (compute-clouds.comp, bit-for-bit). */
@@ -225,6 +227,39 @@ int mm_march_clouds_counted(const mm_camera_ubo* camera, const mm_sun_ubo* sun, 
                             float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
                             int32_t phase, const mm_march_params* params,
                             unsigned long long* counts, void* stream);
+
+/* ---------------------------------------------------------------- many views in one launch
+ * No counterpart in the reference (which dispatches one view per frame, VulkanApplication.cpp:
+ * 1061-1071): many views of any resolution marched by one persistent launch over a global queue of
+ * 64-pixel tiles. Views share the curl and noise volumes and bring their own uniforms, weather map
+ * and output image. Each view's image is identical to mm_march_clouds of that view with the same
+ * params. A plan holds the per-view constants in device memory; mm_march_views enqueues only
+ * kernels (no allocation, no host transfer, no memset), so it can be captured into a HIP graph and
+ * replayed; calls on one plan must not overlap (they share its tile counter). */
+typedef struct mm_view {
+    mm_camera_ubo camera;
+    mm_sun_ubo sun;
+    mm_sky_ubo sky;
+    mm_texture weather;         /* this view's weather map (format of the shared set's weather) */
+    float* out_rgba32f;         /* device, row_pitch_px pixels per row */
+    int32_t width, height, row_pitch_px;
+    int32_t phase;              /* 0..15, MM_PHASE_ALL or MM_PHASE_FROM_UBO */
+} mm_view;
+
+typedef struct mm_view_plan mm_view_plan;
+
+/* shared: curl, low_res and hi_res are used (its weather entry is ignored). params applies to every
+ * view (NULL = reference preset); tile shape from params->tile_width. */
+int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_set* shared,
+                        const mm_march_params* params, mm_view_plan** plan, void* stream);
+/* Replace the views' uniforms / outputs (same count) without reallocating. Synchronises the stream. */
+int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_views, const mm_march_params* params,
+                        void* stream);
+/* Total tiles and the persistent grid (waves) of the plan. */
+int mm_view_plan_info(const mm_view_plan* plan, int32_t* total_tiles, int32_t* grid_waves);
+/* counts: NULL or device uint64[4] accumulators as for mm_march_clouds_counted. */
+int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* stream);
+void mm_view_plan_destroy(mm_view_plan* plan);
 
 /* ---------------------------------------------------------------- learned light-cone surrogate
  * No counterpart in the reference: an optional replacement of the six-sample light cone

@@ -90,7 +90,7 @@ class MarchParams(ctypes.Structure):
                 ("wind_strength", ctypes.c_float), ("weather_scale", ctypes.c_float), ("low_scale", ctypes.c_float),
                 ("curl_scale", ctypes.c_float), ("hi_scale", ctypes.c_float), ("subtexel_bits", ctypes.c_int32),
                 ("cone_mode", ctypes.c_int32), ("surrogate_threshold", ctypes.c_float),
-                ("surrogate", ctypes.c_void_p), ("_reserved", ctypes.c_int32 * 8)]
+                ("surrogate", ctypes.c_void_p), ("tile_width", ctypes.c_int32), ("_reserved", ctypes.c_int32 * 7)]
 
 
 class Scene(ctypes.Structure):
@@ -99,6 +99,12 @@ class Scene(ctypes.Structure):
                 ("sun_azimuth", ctypes.c_float), ("time", ctypes.c_float), ("wind", ctypes.c_float * 3),
                 ("rayleigh", ctypes.c_float), ("mie", ctypes.c_float), ("mie_directional", ctypes.c_float),
                 ("turbidity", ctypes.c_float), ("phase", ctypes.c_int32)]
+
+
+class View(ctypes.Structure):
+    _fields_ = [("camera", CameraUBO), ("sun", SunUBO), ("sky", SkyUBO), ("weather", Texture),
+                ("out_rgba32f", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("row_pitch_px", ctypes.c_int32), ("phase", ctypes.c_int32)]
 
 
 # Every symbol include/mm.h declares, with its ctypes signature.
@@ -119,6 +125,13 @@ _SIGS = {
     "mm_march_clouds_counted": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
                                                ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                ctypes.c_int32, _P(MarchParams), ctypes.c_void_p, ctypes.c_void_p]),
+    "mm_view_plan_create": (ctypes.c_int, [_P(View), ctypes.c_int32, _P(NoiseSetC), _P(MarchParams),
+                                           _P(ctypes.c_void_p), ctypes.c_void_p]),
+    "mm_view_plan_update": (ctypes.c_int, [ctypes.c_void_p, _P(View), ctypes.c_int32, _P(MarchParams),
+                                           ctypes.c_void_p]),
+    "mm_view_plan_info": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), _P(ctypes.c_int32)]),
+    "mm_march_views": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "mm_view_plan_destroy": (None, [ctypes.c_void_p]),
     "mm_surrogate_bytes": (ctypes.c_size_t, []),
     "mm_surrogate_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
     "mm_surrogate_fit_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
@@ -148,6 +161,12 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"libmm.so not built: {LIB_PATH} (run __graft_entry__.build())")
+        try:
+            # One HIP runtime per process: PyTorch-ROCm bundles its own libamdhip64.so.7; loading it
+            # first lets libmm.so bind to that same runtime (streams and allocations are shared).
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(l, name)
@@ -390,3 +409,51 @@ def march_collect(cam: CameraUBO, sun: SunUBO, sky: SkyUBO, noise: NoiseSet, out
                                          samples.data_ptr(), samples.shape[0], keep_one_in, seed, count.data_ptr(),
                                          stream)
     check(code, "mm_march_clouds_collect")
+
+
+# ---------------------------------------------------------------- many views in one launch
+class ViewPlan:
+    """Per-view constants in device memory for one persistent multi-view launch (mm_view_plan)."""
+
+    def __init__(self, views, shared: NoiseSet, params: MarchParams | None = None, stream: int | None = None):
+        self.views = (View * len(views))(*views)
+        self.n = len(views)
+        self.shared = shared            # keeps the shared textures alive
+        stream = current_stream() if stream is None else stream
+        h = ctypes.c_void_p()
+        pp = ctypes.byref(params) if params is not None else None
+        check(lib().mm_view_plan_create(self.views, self.n, ctypes.byref(shared.c), pp, ctypes.byref(h), stream),
+              "mm_view_plan_create")
+        self.handle = h
+
+    def update(self, views, params: MarchParams | None = None, stream: int | None = None) -> None:
+        self.views = (View * len(views))(*views)
+        stream = current_stream() if stream is None else stream
+        pp = ctypes.byref(params) if params is not None else None
+        check(lib().mm_view_plan_update(self.handle, self.views, len(views), pp, stream), "mm_view_plan_update")
+
+    def info(self):
+        t, g = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().mm_view_plan_info(self.handle, ctypes.byref(t), ctypes.byref(g)), "mm_view_plan_info")
+        return t.value, g.value
+
+    def launch(self, counts=None, stream: int | None = None) -> None:
+        stream = current_stream() if stream is None else stream
+        check(lib().mm_march_views(self.handle, counts.data_ptr() if counts is not None else None, stream),
+              "mm_march_views")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().mm_view_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_view(cam: CameraUBO, sun: SunUBO, sky: SkyUBO, weather: DeviceTexture, out, width: int, height: int,
+              phase: int = PHASE_ALL, pitch: int | None = None) -> View:
+    return View(cam, sun, sky, weather.desc, out.data_ptr(), width, height, width if pitch is None else pitch, phase)

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
+#include <new>
+#include <vector>
 #include "mm_internal.hpp"
 
 namespace {
@@ -104,6 +106,7 @@ int check_params(const mm_march_params* p) {
     if (p->miss_limit < 1) return MM_E_PARAM;
     if (p->subtexel_bits != 0 && p->subtexel_bits != 8) return MM_E_PARAM;
     if (p->cone_mode != MM_CONE_PLAIN && p->cone_mode != MM_CONE_SURROGATE) return MM_E_PARAM;
+    if (p->tile_width != 0 && p->tile_width != 8 && p->tile_width != 16 && p->tile_width != 32) return MM_E_PARAM;
     if (p->cone_mode == MM_CONE_SURROGATE && (!p->surrogate || !(p->surrogate_threshold >= 0.f)))
         return MM_E_PARAM;
     return MM_OK;
@@ -249,10 +252,13 @@ struct CollectArgs {
     uint32_t* count;
 };
 
-static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
-                      const mm_noise_set* tex, const mm_texture* night_sky, float* out, int32_t width, int32_t height,
-                      int32_t pitch, int32_t phase, const mm_march_params* params, unsigned long long* counts,
-                      const CollectArgs* collect, void* stream) {
+// Validates one dispatch and derives its per-dispatch constants and texture views. `empty` is set
+// when the phase selects no pixel of the image (nothing to launch).
+static int build_march(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                       const mm_noise_set* tex, const mm_texture* night_sky, const float* out, int32_t width,
+                       int32_t height, int32_t pitch, int32_t phase, const mm_march_params* params,
+                       const CollectArgs* collect, mm::MarchConsts& c, mm::MarchTextures& t, int& fset_out,
+                       bool& empty) {
     if (!camera || !sun || !sky || !tex || !out) return MM_E_NULL;
     if (width <= 0 || height <= 0 || width > 32768 || height > 32768) return MM_E_DIM;
     if (pitch < width) return MM_E_PITCH;
@@ -293,7 +299,6 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
             return MM_E_SIZE;
     }
 
-    mm::MarchConsts c;
     memset(&c, 0, sizeof(c));
     const float* V = camera->view;
     for (int k = 0; k < 3; k++) {
@@ -389,12 +394,12 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
         c.collectSeed = collect->seed;
         c.collectCount = collect->count;
     }
-    if (c.gridW <= 0 || c.gridH <= 0) return MM_OK;  // no pixel of this phase in the image
-    c.tilesX = (c.gridW + 7) / 8;
-    c.tilesY = (c.gridH + 7) / 8;
+    empty = c.gridW <= 0 || c.gridH <= 0;  // no pixel of this phase in the image
+    c.tileShift = params->tile_width == 16 ? 4 : params->tile_width == 32 ? 5 : 3;
+    c.tilesX = (c.gridW + (1 << c.tileShift) - 1) >> c.tileShift;
+    c.tilesY = (c.gridH + (64 >> c.tileShift) - 1) / (64 >> c.tileShift);
     c.tilesPerXcd = (c.tilesX * c.tilesY + 7) / 8;
 
-    mm::MarchTextures t;
     memset(&t, 0, sizeof(t));
     t.weather = view_of(tex->weather);
     t.curl = view_of(tex->curl);
@@ -404,6 +409,21 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
         t.night = view_of(*night_sky);
         t.hasNight = 1;
     }
+    fset_out = fset;
+    return MM_OK;
+}
+
+static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                      const mm_noise_set* tex, const mm_texture* night_sky, float* out, int32_t width, int32_t height,
+                      int32_t pitch, int32_t phase, const mm_march_params* params, unsigned long long* counts,
+                      const CollectArgs* collect, void* stream) {
+    mm::MarchConsts c;
+    mm::MarchTextures t;
+    int fset = 0;
+    bool empty = false;
+    const int e = build_march(camera, sun, sky, tex, night_sky, out, width, height, pitch, phase, params, collect, c,
+                              t, fset, empty);
+    if (e || empty) return e;
     reset_error_state();
     return mm::launch_march(fset, c, t, out, counts, stream);
 }
@@ -435,6 +455,141 @@ int mm_march_clouds_collect(const mm_camera_ubo* camera, const mm_sun_ubo* sun, 
     CollectArgs a{samples, capacity, keep_one_in, seed, count};
     return march_impl(camera, sun, sky, textures, night_sky, out_rgba32f, width, height, row_pitch_px, phase, params,
                       nullptr, &a, stream);
+}
+
+struct mm_view_plan {
+    int n_views = 0, total_tiles = 0, grid = 0, fset = 0, mode = 0;
+    mm::ViewDesc* d_views = nullptr;  // device: ViewDesc[n_views] then the tile counter
+    unsigned* d_counter = nullptr;
+    mm::MarchTextures shared;
+    mm_noise_set shared_set;          // the shared textures as given (curl, low_res, hi_res)
+    int device = 0;
+};
+
+static int fill_views(const mm_view* views, int32_t n, const mm_noise_set* shared, const mm_march_params* params,
+                      std::vector<mm::ViewDesc>& out, mm::MarchTextures& tx, int& fset, int& total) {
+    out.resize((size_t)n);
+    total = 0;
+    fset = -1;
+    for (int i = 0; i < n; i++) {
+        const mm_view& v = views[i];
+        mm_noise_set set = *shared;
+        set.weather = v.weather;
+        mm::MarchConsts c;
+        mm::MarchTextures t;
+        int fs = 0;
+        bool empty = false;
+        const int e = build_march(&v.camera, &v.sun, &v.sky, &set, nullptr, v.out_rgba32f, v.width, v.height,
+                                  v.row_pitch_px, v.phase, params, nullptr, c, t, fs, empty);
+        if (e) return e;
+        if (fset >= 0 && fs != fset) return MM_E_FORMAT;
+        fset = fs;
+        if (i == 0) tx = t;
+        mm::ViewDesc& d = out[(size_t)i];
+        memset(&d, 0, sizeof(d));
+        d.c = c;
+        d.weather = t.weather;
+        d.out = v.out_rgba32f;
+        d.tileBase = total;
+        const long long tiles = empty ? 0 : (long long)c.tilesX * c.tilesY;
+        if ((long long)total + tiles > (1ll << 30)) return MM_E_DIM;
+        total += (int)tiles;
+    }
+    return MM_OK;
+}
+
+int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_set* shared,
+                        const mm_march_params* params, mm_view_plan** plan, void* stream) {
+    if (!views || !shared || !plan) return MM_E_NULL;
+    *plan = nullptr;
+    if (n_views < 1 || n_views > 65536) return MM_E_PARAM;
+    mm_march_params ref;
+    if (!params) {
+        mm_march_params_reference(&ref);
+        params = &ref;
+    }
+    std::vector<mm::ViewDesc> h;
+    mm::MarchTextures tx;
+    int fset = 0, total = 0;
+    int e = fill_views(views, n_views, shared, params, h, tx, fset, total);
+    if (e) return e;
+    reset_error_state();
+    mm_view_plan* p = new (std::nothrow) mm_view_plan();
+    if (!p) return MM_E_DEVICE;
+    p->n_views = n_views;
+    p->total_tiles = total;
+    p->fset = fset;
+    p->mode = params->cone_mode;
+    p->shared = tx;
+    p->shared_set = *shared;
+    (void)hipGetDevice(&p->device);
+    if ((e = mm::views_resident_waves(fset, p->mode, &p->grid))) {
+        delete p;
+        return e;
+    }
+    const size_t bytes = sizeof(mm::ViewDesc) * (size_t)n_views;
+    if (hipMalloc((void**)&p->d_views, bytes + 256) != hipSuccess) {
+        delete p;
+        return MM_E_DEVICE;
+    }
+    p->d_counter = (unsigned*)((uint8_t*)p->d_views + ((bytes + 255) / 256) * 256);
+    hipStream_t s = (hipStream_t)stream;
+    if (hipMemcpyAsync(p->d_views, h.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipFree(p->d_views);
+        delete p;
+        return MM_E_DEVICE;
+    }
+    *plan = p;
+    return MM_OK;
+}
+
+int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_views, const mm_march_params* params,
+                        void* stream) {
+    if (!plan || !views) return MM_E_NULL;
+    if (n_views != plan->n_views) return MM_E_PARAM;
+    mm_march_params ref;
+    if (!params) {
+        mm_march_params_reference(&ref);
+        params = &ref;
+    }
+    if (params->cone_mode != plan->mode) return MM_E_PARAM;
+    // the shared volumes stay those of the plan; views bring new uniforms, weather maps and outputs
+    std::vector<mm::ViewDesc> h;
+    mm::MarchTextures tx;
+    int fset = 0, total = 0;
+    int e = fill_views(views, n_views, &plan->shared_set, params, h, tx, fset, total);
+    if (e) return e;
+    if (fset != plan->fset) return MM_E_FORMAT;
+    reset_error_state();
+    hipStream_t s = (hipStream_t)stream;
+    if (hipMemcpyAsync(plan->d_views, h.data(), sizeof(mm::ViewDesc) * (size_t)n_views, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return MM_E_DEVICE;
+    plan->total_tiles = total;
+    return MM_OK;
+}
+
+int mm_view_plan_info(const mm_view_plan* plan, int32_t* total_tiles, int32_t* grid_waves) {
+    if (!plan) return MM_E_NULL;
+    if (total_tiles) *total_tiles = plan->total_tiles;
+    if (grid_waves) *grid_waves = plan->grid;
+    return MM_OK;
+}
+
+int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* stream) {
+    if (!plan) return MM_E_NULL;
+    if (plan->total_tiles == 0) return MM_OK;
+    reset_error_state();
+    return mm::launch_views(plan->fset, plan->mode, plan->d_views, plan->n_views, plan->shared, plan->d_counter,
+                            plan->total_tiles, plan->grid, counts, stream);
+}
+
+void mm_view_plan_destroy(mm_view_plan* plan) {
+    if (!plan) return;
+    (void)hipFree(plan->d_views);
+    delete plan;
 }
 
 size_t mm_surrogate_bytes(void) { return mm::surrogate_model_bytes(); }

@@ -1,8 +1,8 @@
 // Cloud raymarch kernel for gfx950 (CDNA4), the HIP counterpart of the compute dispatch of
 // SkyEngine/SkyEngine/Shaders/compute-clouds.comp:288-500.
 //
-// One work-item per marched pixel, 64-lane wavefronts covering 8x8 pixel tiles (rays of a wave
-// are spatially coherent, so their step counts and texel footprints are close). Uniform-derived
+// One work-item per marched pixel, 64-lane wavefronts covering 64-pixel tiles (8x8, 16x4 or 32x2;
+// rays of a wave are spatially coherent, so their step counts and texel footprints are close). Uniform-derived
 // values are hoisted to the host (MarchConsts). Texture filtering is done in software: LINEAR +
 // REPEAT with the texture unit's fixed-point sub-texel weights (8 bits) or exact float weights,
 // over one of three texel format sets:
@@ -22,6 +22,7 @@ namespace {
 using mm::MarchConsts;
 using mm::MarchTextures;
 using mm::TexView;
+using mm::ViewDesc;
 
 struct V3 {
     float x, y, z;
@@ -725,14 +726,15 @@ __device__ __forceinline__ void tile_pixel(const MarchConsts& c, int gi, int gj,
 }
 
 // ------------------------------------------------------------------ lock-step kernel
-// One wave per 8x8 tile; every lane evaluates its light cones in place (the shader's structure).
+// One wave per 64-pixel tile; every lane evaluates its light cones in place (the shader's structure).
 template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
                                                    unsigned long long* __restrict__ counts) {
     const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
     if (tile >= c.tilesX * c.tilesY) return;
     const int lane = threadIdx.x;
-    const int gi = (tile % c.tilesX) * 8 + (lane & 7), gj = (tile / c.tilesX) * 8 + (lane >> 3);
+    const int gi = (tile % c.tilesX) * (1 << c.tileShift) + (lane & ((1 << c.tileShift) - 1));
+    const int gj = (tile / c.tilesX) * (64 >> c.tileShift) + (lane >> c.tileShift);
     if (gi >= c.gridW || gj >= c.gridH) return;
     int px, py;
     tile_pixel(c, gi, gj, px, py);
@@ -765,7 +767,7 @@ __global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const Ma
 }
 
 // ------------------------------------------------------------------ cone-batched kernel
-// One wave per 8x8 tile. Lanes run their primary march in lock-step but hand light-cone work to a
+// One wave per 64-pixel tile. Lanes run their primary march in lock-step but hand light-cone work to a
 // wave-local LDS queue (slots assigned with ballot + mbcnt in creation order); once 64 jobs are
 // queued (or no lane is still marching) the wave evaluates a full batch, one job per lane, and each
 // lane folds its results into its transmittance in creation order. Cone work therefore runs at
@@ -791,13 +793,13 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
 }
 
 template <int FSET, bool COUNT, int MODE>
-__device__ __forceinline__ void march_batched_body(const MarchConsts& c, const MarchTextures& tx,
-                                                   float* __restrict__ out, unsigned long long* __restrict__ counts) {
+__device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchTextures& tx, const int tile,
+                                           float* __restrict__ out, unsigned long long* __restrict__ counts) {
     __shared__ ConeQueue q;
-    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
-    if (tile >= c.tilesX * c.tilesY) return;  // uniform across the block
     const int lane = threadIdx.x;
-    const int gi = (tile % c.tilesX) * 8 + (lane & 7), gj = (tile / c.tilesX) * 8 + (lane >> 3);
+    // tile of (1 << tileShift) x (64 >> tileShift) pixels of the marched grid, lane-major in x
+    const int gi = (tile % c.tilesX) * (1 << c.tileShift) + (lane & ((1 << c.tileShift) - 1));
+    const int gj = (tile / c.tilesX) * (64 >> c.tileShift) + (lane >> c.tileShift);
     const bool valid = gi < c.gridW && gj < c.gridH;
     int px = 0, py = 0;
     if (valid) tile_pixel(c, gi, gj, px, py);
@@ -923,7 +925,9 @@ template <int FSET, bool COUNT, int MODE>
 __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, const MarchTextures tx,
                                                            float* __restrict__ out,
                                                            unsigned long long* __restrict__ counts) {
-    march_batched_body<FSET, COUNT, MODE>(c, tx, out, counts);
+    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
+    if (tile >= c.tilesX * c.tilesY) return;  // uniform across the block
+    march_tile<FSET, COUNT, MODE>(c, tx, tile, out, counts);
 }
 
 #ifndef MM_SURROGATE_WAVES_PER_EU
@@ -935,7 +939,69 @@ template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
 void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
                             unsigned long long* __restrict__ counts) {
-    march_batched_body<FSET, COUNT, CONE_SURROGATE>(c, tx, out, counts);
+    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
+    if (tile >= c.tilesX * c.tilesY) return;
+    march_tile<FSET, COUNT, CONE_SURROGATE>(c, tx, tile, out, counts);
+}
+
+// ------------------------------------------------------------------ multi-view persistent kernel
+// A fixed grid of waves (sized to fill every CU) drains one global queue of 64-pixel tiles that
+// spans all views: each wave claims the next tile with one atomic, finds its view by binary search
+// over the views' first-tile indices, and marches it with that view's constants and weather map.
+// Tiles are numbered view by view, so the tiles in flight at any time are a contiguous range
+// (mostly one view: its weather map stays cache-resident). Every wave leaves the loop once the
+// counter passes the last tile.
+__global__ void reset_counter_kernel(unsigned* ctr) {
+    if (threadIdx.x == 0) *ctr = 0u;
+}
+
+template <int FSET, bool COUNT, int MODE>
+__device__ __forceinline__ void views_loop(const ViewDesc* __restrict__ views, int nViews, MarchTextures tx,
+                                           unsigned* __restrict__ ctr, int total,
+                                           unsigned long long* __restrict__ counts) {
+    for (;;) {
+        unsigned t = 0;
+        if (threadIdx.x == 0) t = atomicAdd(ctr, 1u);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        if (t >= (unsigned)total) break;
+        int lo = 0, hi = nViews - 1;  // last view whose first tile is <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (views[mid].tileBase <= (int)t) lo = mid;
+            else hi = mid - 1;
+        }
+        const ViewDesc& v = views[lo];
+        tx.weather = v.weather;
+        march_tile<FSET, COUNT, MODE>(v.c, tx, (int)t - v.tileBase, v.out, counts);
+    }
+}
+
+template <int FSET, bool COUNT, int MODE>
+__global__ __launch_bounds__(64) void march_views_kernel(const ViewDesc* __restrict__ views, int nViews,
+                                                         const MarchTextures tx, unsigned* __restrict__ ctr, int total,
+                                                         unsigned long long* __restrict__ counts) {
+    views_loop<FSET, COUNT, MODE>(views, nViews, tx, ctr, total, counts);
+}
+
+template <int FSET, bool COUNT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
+void march_views_surrogate_kernel(const ViewDesc* __restrict__ views, int nViews, const MarchTextures tx,
+                                  unsigned* __restrict__ ctr, int total, unsigned long long* __restrict__ counts) {
+    views_loop<FSET, COUNT, CONE_SURROGATE>(views, nViews, tx, ctr, total, counts);
+}
+
+template <int FSET, bool COUNT>
+const void* views_kernel_ptr(int mode) {
+    if (mode == CONE_SURROGATE) {
+        if constexpr (FSET != mm::FSET_PREC) return (const void*)march_views_surrogate_kernel<FSET, COUNT>;
+        return nullptr;
+    }
+    return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN>;
+}
+
+template <int FSET>
+const void* views_kernel_ptr_c(int mode, bool count) {
+    return count ? views_kernel_ptr<FSET, true>(mode) : views_kernel_ptr<FSET, false>(mode);
 }
 
 template <int FSET, bool COUNT>
@@ -959,6 +1025,32 @@ int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned 
 }  // namespace
 
 namespace mm {
+
+int views_resident_waves(int fset, int mode, int* out) {
+    const void* k = fset == FSET_REF ? views_kernel_ptr_c<FSET_REF>(mode, false)
+                    : fset == FSET_FAST ? views_kernel_ptr_c<FSET_FAST>(mode, false)
+                                        : views_kernel_ptr_c<FSET_PREC>(mode, false);
+    if (!k) return MM_E_FORMAT;
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return MM_E_DEVICE;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return MM_E_DEVICE;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 64, 0) != hipSuccess) return MM_E_DEVICE;
+    *out = cus * (per > 0 ? per : 1);
+    return MM_OK;
+}
+
+int launch_views(int fset, int mode, const ViewDesc* views, int nViews, const MarchTextures& t, unsigned* ctr,
+                 int total, int grid, unsigned long long* counts, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const void* k = fset == FSET_REF ? views_kernel_ptr_c<FSET_REF>(mode, counts != nullptr)
+                    : fset == FSET_FAST ? views_kernel_ptr_c<FSET_FAST>(mode, counts != nullptr)
+                                        : views_kernel_ptr_c<FSET_PREC>(mode, counts != nullptr);
+    if (!k) return MM_E_FORMAT;
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, ctr);
+    void* args[] = {(void*)&views, (void*)&nViews, (void*)&t, (void*)&ctr, (void*)&total, (void*)&counts};
+    if (hipLaunchKernel(k, dim3((unsigned)grid), dim3(64), args, 0, s) != hipSuccess) return MM_E_DEVICE;
+    return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
+}
 
 int launch_march(int fset, const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts,
                  void* stream) {

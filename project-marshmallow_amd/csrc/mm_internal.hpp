@@ -35,7 +35,8 @@ struct MarchConsts {
     int width, height, pitch;    // image size used for uv and bounds, row pitch in pixels
     int phase;                   // -1 = all pixels
     int gridW, gridH;            // marched pixel grid
-    int tilesX, tilesY;           // 8x8 tiles of the marched grid
+    int tileShift;                // tiles of (1 << tileShift) x (64 >> tileShift) pixels: 3, 4 or 5
+    int tilesX, tilesY;           // tiles of the marched grid
     int tilesPerXcd;              // ceil(tilesX * tilesY / 8)
     float probeOffsets[3][3];     // basis * (0, k, 0), k = 0.6, 2.5, 6: surrogate density probes
     // light-cone mode: MM_CONE_PLAIN, MM_CONE_SURROGATE, or 2 = plain + record surrogate training samples
@@ -64,6 +65,18 @@ struct MarchTextures {
     TexView weather, curl, low, hi, night;
     int hasNight;
 };
+
+// One view of a multi-view launch (device array, built on the host).
+struct ViewDesc {
+    MarchConsts c;
+    TexView weather;
+    float* out;
+    int tileBase;                 // global index of this view's first tile
+};
+
+int views_resident_waves(int fset, int mode, int* out);
+int launch_views(int fset, int mode, const ViewDesc* views, int nViews, const MarchTextures& t, unsigned* ctr,
+                 int total, int grid, unsigned long long* counts, void* stream);
 
 size_t surrogate_model_bytes();
 int surrogate_init(void* model, uint32_t seed, void* stream);

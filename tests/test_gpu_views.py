@@ -1,0 +1,127 @@
+"""Many views in one persistent launch: each view's image equals its own single-view march, the
+launch replays correctly from a HIP graph, and the tile shape does not change the image."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def vs(mm, gpu):
+    from marshmallow_amd import multiview
+
+    specs = multiview.view_specs(n=6, seed=3, lo=(64, 48), hi=(300, 170))
+    specs.append(multiview.ViewSpec(37, 23, 2500.0, 12.0, 0.6, 77))   # ragged, camera inside the shell
+    return multiview.ViewSet(mm, specs)
+
+
+def _single(mm, vs, prm):
+    import torch
+
+    outs = [torch.zeros_like(o) for o in vs.outs]
+    vs.march_one_by_one(prm, outs)
+    torch.cuda.synchronize()
+    return outs
+
+
+@pytest.mark.parametrize("tile_width", [8, 16, 32])
+def test_views_equal_single_view_marches(mm, vs, tile_width):
+    import torch
+
+    prm = mm.march_params("scenario", tile_width=tile_width)
+    ref = _single(mm, vs, mm.march_params("scenario"))
+    for o in vs.outs:
+        o.zero_()
+    plan = mm.ViewPlan(vs.views(), vs.shared, prm)
+    total, grid = plan.info()
+    assert total == sum(-(-s.width // tile_width) * -(-s.height // (64 // tile_width)) for s in vs.specs)
+    assert grid > 0
+    cnt = torch.zeros(4, dtype=torch.int64, device="cuda")
+    plan.launch(cnt)
+    torch.cuda.synchronize()
+    for i, (a, b) in enumerate(zip(vs.outs, ref)):
+        assert torch.equal(a, b), (i, vs.specs[i])
+    assert int(cnt[0]) > 0 and int(cnt[2]) == int(cnt[3])
+    plan.close()
+
+
+def test_views_phase_and_update(mm, vs):
+    import torch
+
+    prm = mm.march_params("scenario")
+    plan = mm.ViewPlan(vs.views(phase=5), vs.shared, prm)
+    for o in vs.outs:
+        o.fill_(-1.0)
+    plan.launch()
+    torch.cuda.synchronize()
+    for i, s in enumerate(vs.specs):
+        img = vs.outs[i].view(s.height, s.width, 4)
+        m = torch.zeros(s.height, s.width, dtype=torch.bool, device="cuda")
+        m[1::4, 1::4] = True  # phase 5 -> (x % 4, y % 4) = (1, 1)
+        assert (img[~m] == -1.0).all()
+        if m.any():
+            assert (img[m][:, 3] >= 0).all()
+    # update to all pixels: same tiles as a fresh plan
+    plan.update(vs.views(), prm)
+    t2, _ = plan.info()
+    assert t2 == mm.ViewPlan(vs.views(), vs.shared, prm).info()[0]
+    plan.close()
+
+
+def test_views_graph_replay(mm, vs):
+    """The launch captured into a HIP graph: every replay (with eager work and syncs in between)
+    reproduces the eager result."""
+    import torch
+
+    from marshmallow_amd import multiview
+
+    prm = mm.march_params("scenario", tile_width=16)
+    plan = mm.ViewPlan(vs.views(), vs.shared, prm)
+    plan.launch()
+    torch.cuda.synchronize()
+    eager = [o.clone() for o in vs.outs]
+    g = multiview.capture(plan)
+    for rep in range(3):
+        for o in vs.outs:
+            o.zero_()
+        vs.march_one_by_one(mm.march_params("scenario"), [torch.empty_like(o) for o in vs.outs])  # eager work
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for i, (a, b) in enumerate(zip(vs.outs, eager)):
+            assert torch.equal(a, b), (rep, i)
+    plan.close()
+
+
+def test_views_surrogate_threshold_zero(mm, vs):
+    import torch
+
+    ref = _single(mm, vs, mm.march_params("scenario"))
+    model = mm.Surrogate(seed=2)
+    prm = mm.march_params("scenario", cone_mode=mm.CONE_SURROGATE, surrogate_threshold=0.0, surrogate=model.ptr)
+    plan = mm.ViewPlan(vs.views(), vs.shared, prm)
+    plan.launch()
+    torch.cuda.synchronize()
+    for a, b in zip(vs.outs, ref):
+        assert torch.equal(a, b)
+    plan.close()
+
+
+def test_tile_shape_does_not_change_the_image(mm, gpu):
+    import torch
+
+    W, H = 200, 120
+    ns = mm.NoiseSet("fast")
+    ns.generate(mm.noise_params(seed=1, weather_seed=3, coverage=0.6), mm.current_stream())
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H))
+    outs = []
+    for tw in (8, 16, 32):
+        for flags in (0, mm.MARCH_NO_BATCH):
+            prm = mm.march_params("scenario", tile_width=tw)
+            prm.flags |= flags
+            o = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+            mm.march_clouds(cam, sun, sky, ns, o, W, H, params=prm)
+            outs.append(o)
+    torch.cuda.synchronize()
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
