@@ -44,7 +44,9 @@ const char* mm_strerror(int code);
 int mm_abi_version(void);
 
 /* ---------------------------------------------------------------- uniform blocks (std140) */
-/* UniformCameraObject: Shader.h:24-28, compute-clouds.comp:12-17. 160 bytes.
+/* UniformCameraObject: Shader.h:24-28, compute-clouds.comp:12-17. 160 bytes. The previous frame's
+ * camera travels in the same host struct (VulkanApplication.cpp:354-358); the shaders' block
+ * UniformCameraObjectPrev (compute-clouds.comp:19-23, reproject.comp:19-23) is its first 144 bytes.
  * view and proj are column-major 4x4 (glm/GLSL order: m[col*4 + row]).
  * cameraParams = (aspect, tan(0.5 * DEG2RAD * fov), 0, 0) (VulkanApplication.cpp:365-366). */
 typedef struct mm_camera_ubo {
@@ -227,6 +229,27 @@ int mm_march_clouds_counted(const mm_camera_ubo* camera, const mm_sun_ubo* sun, 
                             float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
                             int32_t phase, const mm_march_params* params,
                             unsigned long long* counts, void* stream);
+
+/* ---------------------------------------------------------------- temporal reprojection
+ * Replaces the dispatch of reproject.comp (VulkanApplication.cpp:1051-1057; the shader body is
+ * reproject.comp:91-153): every target pixel is fetched from the previous frame `source` along the
+ * pixel's motion vector between camera_prev and camera (10 nearest taps averaged, alpha from the
+ * centre tap), the motion being that of the ray's hit on the inner shell sphere. params gives the
+ * sphere (inner_diameter, center_y, MM_MARCH_ISECT_FP64); NULL = the reference preset, whose sphere
+ * is reproject.comp's ATMOSPHERE_RADIUS 2e6. source and target: RGBA32F, width x height,
+ * row_pitch_px pixels per row; they must not overlap. GLSL latitude fixed here: round() is half
+ * away from zero; a NaN coordinate reads texel 0. */
+int mm_reproject(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, const float* source,
+                 float* target, int32_t width, int32_t height, int32_t row_pitch_px, const mm_march_params* params,
+                 void* stream);
+
+/* One frame of the reference's compute command buffer (VulkanApplication.cpp:1046-1071):
+ * mm_reproject(source -> target) then the cloud march of the pixel phase int(sun.color.a) into
+ * target. The caller swaps source and target for the next frame. */
+int mm_render_frame(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, const mm_sun_ubo* sun,
+                    const mm_sky_ubo* sky, const mm_noise_set* textures, const mm_texture* night_sky,
+                    const float* source, float* target, int32_t width, int32_t height, int32_t row_pitch_px,
+                    const mm_march_params* params, void* stream);
 
 /* ---------------------------------------------------------------- many views in one launch
  * No counterpart in the reference (which dispatches one view per frame, VulkanApplication.cpp:

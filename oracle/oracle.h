@@ -8,6 +8,7 @@
  *                     ships baked assets, README.md:68, so these follow the build's own spec)
  *   - orc_march_*     compute-clouds.comp:65-500 (the cloud march), with the sampler semantics of
  *                     Texture.cpp:29-52, 315-338 (LINEAR filter, REPEAT wrap, RGBA8 UNORM)
+ *   - orc_reproject   reproject.comp:56-153 (temporal reprojection of the previous frame)
  * Built with -ffp-contract=off and no fast-math so every float operation rounds as written.
  */
 #ifndef MM_ORACLE_H
@@ -80,6 +81,12 @@ int orc_march_pixels_f64(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const 
                          const int32_t* xy, int n, double* out, int threads);
 
 int orc_max_threads(void);
+
+/* ---- temporal reprojection (reproject.comp) ---- */
+/* Target rows [y0, y1) from the previous frame src (both RGBA32F, pitch pixels per row). prm gives
+ * the inner sphere (inner_diameter, center_y) and the FP64 flag; it must not be NULL. */
+int orc_reproject(const mm_camera_ubo* cam, const mm_camera_ubo* prev, const float* src, float* dst,
+                  int width, int height, int pitch, const mm_march_params* prm, int y0, int y1);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,7 @@ class Oracle:
             "orc_remap_clamped": (f, [f, f, f, f, f]),
             "orc_ray_sphere_f32": (f, [vp, vp, vp, f, vp]),
             "orc_ray_sphere_f64": (d, [vp, vp, vp, d, vp]),
+            "orc_reproject": (i, [vp, vp, vp, vp, i, i, i, vp, i, i]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -134,6 +135,16 @@ class Oracle:
         if r != 0:
             raise RuntimeError(f"oracle march failed: {r}")
         return (out, cnt) if counts else out
+
+    def reproject(self, cam, cam_prev, src: np.ndarray, params, width: int, height: int) -> np.ndarray:
+        """reproject.comp restated: the target image (height, width, 4) float32 from src."""
+        src = np.ascontiguousarray(src, dtype=np.float32).reshape(height, width, 4)
+        out = np.zeros((height, width, 4), np.float32)
+        r = self.L.orc_reproject(ctypes.addressof(cam), ctypes.addressof(cam_prev), _ptr(src), _ptr(out), width,
+                                 height, width, ctypes.addressof(params), 0, height)
+        if r != 0:
+            raise RuntimeError(f"oracle reproject failed: {r}")
+        return out
 
     def march_pixels(self, cam, sun, sky, tex: OrcTextures, params, width: int, height: int, xy: np.ndarray,
                      f64: bool = False, threads: int = 0):

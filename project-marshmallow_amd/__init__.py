@@ -125,6 +125,11 @@ _SIGS = {
     "mm_march_clouds_counted": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
                                                ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                ctypes.c_int32, _P(MarchParams), ctypes.c_void_p, ctypes.c_void_p]),
+    "mm_reproject": (ctypes.c_int, [_P(CameraUBO), _P(CameraUBO), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.c_int32, _P(MarchParams), ctypes.c_void_p]),
+    "mm_render_frame": (ctypes.c_int, [_P(CameraUBO), _P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC),
+                                       _P(Texture), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, _P(MarchParams), ctypes.c_void_p]),
     "mm_view_plan_create": (ctypes.c_int, [_P(View), ctypes.c_int32, _P(NoiseSetC), _P(MarchParams),
                                            _P(ctypes.c_void_p), ctypes.c_void_p]),
     "mm_view_plan_update": (ctypes.c_int, [ctypes.c_void_p, _P(View), ctypes.c_int32, _P(MarchParams),
@@ -330,6 +335,57 @@ class ComputeShader:
         march_clouds(self.cam, self.sun, self.sky, self.noise, target, self.width, self.height,
                      phase=PHASE_FROM_UBO, params=self.params, night=self.night, stream=stream)
         return target
+
+
+def reproject(cam: CameraUBO, cam_prev: CameraUBO, source, target, width: int, height: int,
+              params: MarchParams | None = None, pitch: int | None = None, stream: int | None = None) -> None:
+    """reproject.comp's pass: target (float32 device tensor) from the previous frame source."""
+    pitch = width if pitch is None else pitch
+    stream = current_stream() if stream is None else stream
+    pp = ctypes.byref(params) if params is not None else None
+    check(lib().mm_reproject(ctypes.byref(cam), ctypes.byref(cam_prev), source.data_ptr(), target.data_ptr(), width,
+                             height, pitch, pp, stream), "mm_reproject")
+
+
+def previous_camera(cam: CameraUBO) -> CameraUBO:
+    """The next frame's UniformCameraObjectPrev: this frame's view, proj and position
+    (VulkanApplication.cpp:354-358)."""
+    p = CameraUBO()
+    ctypes.memmove(ctypes.byref(p), ctypes.byref(cam), ctypes.sizeof(CameraUBO))
+    return p
+
+
+class CloudFrames:
+    """The reference's per-frame compute work (VulkanApplication.cpp:1046-1071, 372-388): the pixel
+    phase advances by one each frame (sun.color.a = (a + 1) % 16), the previous frame is reprojected
+    into the current target at full resolution, then 1/16 of the pixels are marched into it; the
+    two RGBA32F images swap every frame."""
+
+    def __init__(self, width: int, height: int, noise: NoiseSet, params: MarchParams | None = None,
+                 night: DeviceTexture | None = None, device="cuda"):
+        import torch
+
+        self.width, self.height, self.noise, self.params, self.night = width, height, noise, params, night
+        self.images = [torch.zeros(width * height * 4, dtype=torch.float32, device=device) for _ in range(2)]
+        self.cur = 0
+        self.phase = -1
+        self.cam_prev = None
+
+    def frame(self, cam: CameraUBO, sun: SunUBO, sky: SkyUBO, stream: int | None = None):
+        """Render one frame; returns the target image (float32 tensor, height*width*4)."""
+        self.phase = (self.phase + 1) % 16
+        sun.color[3] = float(self.phase)
+        prev = self.cam_prev if self.cam_prev is not None else previous_camera(cam)
+        src, dst = self.images[self.cur], self.images[1 - self.cur]
+        stream = current_stream() if stream is None else stream
+        pp = ctypes.byref(self.params) if self.params is not None else None
+        nt = ctypes.byref(self.night.desc) if self.night is not None else None
+        check(lib().mm_render_frame(ctypes.byref(cam), ctypes.byref(prev), ctypes.byref(sun), ctypes.byref(sky),
+                                    ctypes.byref(self.noise.c), nt, src.data_ptr(), dst.data_ptr(), self.width,
+                                    self.height, self.width, pp, stream), "mm_render_frame")
+        self.cur = 1 - self.cur
+        self.cam_prev = previous_camera(cam)
+        return dst
 
 
 # ---------------------------------------------------------------- learned light-cone surrogate

@@ -457,6 +457,56 @@ int mm_march_clouds_collect(const mm_camera_ubo* camera, const mm_sun_ubo* sun, 
                       nullptr, &a, stream);
 }
 
+int mm_reproject(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, const float* source,
+                 float* target, int32_t width, int32_t height, int32_t row_pitch_px, const mm_march_params* params,
+                 void* stream) {
+    if (!camera || !camera_prev || !source || !target) return MM_E_NULL;
+    if (width <= 0 || height <= 0 || width > 32768 || height > 32768) return MM_E_DIM;
+    if (row_pitch_px < width) return MM_E_PITCH;
+    if (source == target) return MM_E_PARAM;
+    mm_march_params ref;
+    if (!params) {
+        mm_march_params_reference(&ref);
+        params = &ref;
+    }
+    int e = check_params(params);
+    if (e) return e;
+    if (!finite_n(camera->view, 16) || !finite_n(camera->cameraPosition, 4) || !finite_n(camera->cameraParams, 2) ||
+        !finite_n(camera_prev->view, 16))
+        return MM_E_PARAM;
+    if (!(camera->cameraParams[0] != 0.f) || !(camera->cameraParams[1] != 0.f)) return MM_E_PARAM;
+    mm::ReprojConsts c;
+    memset(&c, 0, sizeof(c));
+    const float* V = camera->view;
+    for (int k = 0; k < 3; k++) {
+        c.camPos[k] = camera->cameraPosition[k];
+        c.camRight[k] = V[4 * k + 0];
+        c.camUp[k] = V[4 * k + 1];
+        c.refPoint[k] = camera->cameraPosition[k] - V[4 * k + 2];  // cameraPos - camLook
+    }
+    c.aspect = camera->cameraParams[0];
+    c.tanH = camera->cameraParams[1];
+    memcpy(c.prevView, camera_prev->view, sizeof(c.prevView));
+    c.centerY = params->center_y;
+    c.innerDiam = params->inner_diameter;
+    c.fp64 = (params->flags & MM_MARCH_ISECT_FP64) ? 1 : 0;
+    c.width = width;
+    c.height = height;
+    c.pitch = row_pitch_px;
+    reset_error_state();
+    return mm::launch_reproject(c, source, target, stream);
+}
+
+int mm_render_frame(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, const mm_sun_ubo* sun,
+                    const mm_sky_ubo* sky, const mm_noise_set* textures, const mm_texture* night_sky,
+                    const float* source, float* target, int32_t width, int32_t height, int32_t row_pitch_px,
+                    const mm_march_params* params, void* stream) {
+    int e = mm_reproject(camera, camera_prev, source, target, width, height, row_pitch_px, params, stream);
+    if (e) return e;
+    return march_impl(camera, sun, sky, textures, night_sky, target, width, height, row_pitch_px, MM_PHASE_FROM_UBO,
+                      params, nullptr, nullptr, stream);
+}
+
 struct mm_view_plan {
     int n_views = 0, total_tiles = 0, grid = 0, fset = 0, mode = 0;
     mm::ViewDesc* d_views = nullptr;  // device: ViewDesc[n_views] then the tile counter

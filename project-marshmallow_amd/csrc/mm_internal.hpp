@@ -66,6 +66,17 @@ struct MarchTextures {
     int hasNight;
 };
 
+// Constants of one reprojection pass (reproject.comp:91-153), derived on the host.
+struct ReprojConsts {
+    float camPos[3], camRight[3], camUp[3], refPoint[3];
+    float aspect, tanH;
+    float prevView[16];
+    float centerY, innerDiam;
+    int fp64;
+    int width, height, pitch;
+};
+int launch_reproject(const ReprojConsts& c, const float* src, float* dst, void* stream);
+
 // One view of a multi-view launch (device array, built on the host).
 struct ViewDesc {
     MarchConsts c;
