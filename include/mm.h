@@ -251,6 +251,34 @@ int mm_render_frame(const mm_camera_ubo* camera, const mm_camera_ubo* camera_pre
                     const float* source, float* target, int32_t width, int32_t height, int32_t row_pitch_px,
                     const mm_march_params* params, void* stream);
 
+/* ---------------------------------------------------------------- cloud shadows on surfaces
+ * Replaces the shadow march inside the mesh pass (model.frag:240-283 with its own cloud functions,
+ * model.frag:52-129): for each world-space point, NUM_SHADOW_STEPS low-res density samples toward
+ * the sun, keeping the maximum (the mesh colour is multiplied by 1 - 2 * result, model.frag:282).
+ * Textures: the march's weather and low-res volume (REF or FAST formats). points: device float
+ * [n][3] world positions (fragPositionWC); out: device float [n]. */
+typedef struct mm_shadow_params {
+    uint32_t size;              /* sizeof(mm_shadow_params) */
+    uint32_t flags;             /* MM_MARCH_ISECT_FP64 */
+    float inner_diameter;       /* ATMOSPHERE_RADIUS = 1e6 (model.frag:61) */
+    float center_y;             /* -ATMOSPHERE_RADIUS * 0.5 * 0.99 (model.frag:247) */
+    float thickness;            /* 0.5 * ATMOSPHERE_RADIUS * 0.02 (model.frag:249) */
+    float proj_radius;          /* 0.5 * ATMOSPHERE_RADIUS (model.frag:75) */
+    float step;                 /* 0.1 * thickness (model.frag:256) */
+    int32_t steps;              /* NUM_SHADOW_STEPS = 6 */
+    float origin_scale;         /* shadowRayOrigin = fragPositionWC * 4 (model.frag:260) */
+    float wind_strength;        /* 20 */
+    float weather_scale;        /* 0.00001 (model.frag:110) */
+    float low_scale;            /* 0.000057 (model.frag:113) */
+    int32_t subtexel_bits;      /* 8 = fixed-point sampler weights, 0 = float */
+    int32_t _reserved[7];
+} mm_shadow_params;
+
+void mm_shadow_params_reference(mm_shadow_params* p);
+int mm_cloud_shadow(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                    const mm_texture* weather, const mm_texture* low_res, const float* points, int32_t n,
+                    float* out, const mm_shadow_params* params, void* stream);
+
 /* ---------------------------------------------------------------- many views in one launch
  * No counterpart in the reference (which dispatches one view per frame, VulkanApplication.cpp:
  * 1061-1071): many views of any resolution marched by one persistent launch over a global queue of

@@ -9,6 +9,7 @@
  *   - orc_march_*     compute-clouds.comp:65-500 (the cloud march), with the sampler semantics of
  *                     Texture.cpp:29-52, 315-338 (LINEAR filter, REPEAT wrap, RGBA8 UNORM)
  *   - orc_reproject   reproject.comp:56-153 (temporal reprojection of the previous frame)
+ *   - orc_cloud_shadow model.frag:52-129, 240-283 (the mesh pass's cloud-shadow march)
  * Built with -ffp-contract=off and no fast-math so every float operation rounds as written.
  */
 #ifndef MM_ORACLE_H
@@ -87,6 +88,13 @@ int orc_max_threads(void);
  * the inner sphere (inner_diameter, center_y) and the FP64 flag; it must not be NULL. */
 int orc_reproject(const mm_camera_ubo* cam, const mm_camera_ubo* prev, const float* src, float* dst,
                   int width, int height, int pitch, const mm_march_params* prm, int y0, int y1);
+
+/* ---- cloud shadows (model.frag) ---- */
+/* RGBA8 linear weather (ww x wh) and low-res (ln^3); points [n][3]; out [n]. */
+float orc_shadow_layer_density(float h, float type);   /* model.frag:88-98 */
+int orc_cloud_shadow(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky, const uint8_t* weather,
+                     int ww, int wh, const uint8_t* low, int ln, const float* points, int n, float* out,
+                     const mm_shadow_params* p);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,8 @@ class Oracle:
             "orc_ray_sphere_f32": (f, [vp, vp, vp, f, vp]),
             "orc_ray_sphere_f64": (d, [vp, vp, vp, d, vp]),
             "orc_reproject": (i, [vp, vp, vp, vp, i, i, i, vp, i, i]),
+            "orc_cloud_shadow": (i, [vp, vp, vp, vp, i, i, vp, i, vp, i, vp, vp]),
+            "orc_shadow_layer_density": (f, [f, f]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -144,6 +146,19 @@ class Oracle:
                                  height, width, ctypes.addressof(params), 0, height)
         if r != 0:
             raise RuntimeError(f"oracle reproject failed: {r}")
+        return out
+
+    def cloud_shadow(self, cam, sun, sky, weather: np.ndarray, low: np.ndarray, points: np.ndarray, params):
+        """model.frag's shadow march restated: RGBA8 weather (h, w, 4) and low-res (n, n, n, 4)."""
+        weather = np.ascontiguousarray(weather, np.uint8)
+        low = np.ascontiguousarray(low, np.uint8)
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        out = np.zeros(len(pts), np.float32)
+        r = self.L.orc_cloud_shadow(ctypes.addressof(cam), ctypes.addressof(sun), ctypes.addressof(sky), _ptr(weather),
+                                    weather.shape[1], weather.shape[0], _ptr(low), low.shape[0], _ptr(pts), len(pts),
+                                    _ptr(out), ctypes.addressof(params))
+        if r != 0:
+            raise RuntimeError(f"oracle cloud shadow failed: {r}")
         return out
 
     def march_pixels(self, cam, sun, sky, tex: OrcTextures, params, width: int, height: int, xy: np.ndarray,

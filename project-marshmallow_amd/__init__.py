@@ -101,6 +101,14 @@ class Scene(ctypes.Structure):
                 ("turbidity", ctypes.c_float), ("phase", ctypes.c_int32)]
 
 
+class ShadowParams(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("inner_diameter", ctypes.c_float),
+                ("center_y", ctypes.c_float), ("thickness", ctypes.c_float), ("proj_radius", ctypes.c_float),
+                ("step", ctypes.c_float), ("steps", ctypes.c_int32), ("origin_scale", ctypes.c_float),
+                ("wind_strength", ctypes.c_float), ("weather_scale", ctypes.c_float), ("low_scale", ctypes.c_float),
+                ("subtexel_bits", ctypes.c_int32), ("_reserved", ctypes.c_int32 * 7)]
+
+
 class View(ctypes.Structure):
     _fields_ = [("camera", CameraUBO), ("sun", SunUBO), ("sky", SkyUBO), ("weather", Texture),
                 ("out_rgba32f", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
@@ -130,6 +138,10 @@ _SIGS = {
     "mm_render_frame": (ctypes.c_int, [_P(CameraUBO), _P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC),
                                        _P(Texture), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_int32, _P(MarchParams), ctypes.c_void_p]),
+    "mm_shadow_params_reference": (None, [_P(ShadowParams)]),
+    "mm_cloud_shadow": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(Texture), _P(Texture),
+                                       ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, _P(ShadowParams),
+                                       ctypes.c_void_p]),
     "mm_view_plan_create": (ctypes.c_int, [_P(View), ctypes.c_int32, _P(NoiseSetC), _P(MarchParams),
                                            _P(ctypes.c_void_p), ctypes.c_void_p]),
     "mm_view_plan_update": (ctypes.c_int, [ctypes.c_void_p, _P(View), ctypes.c_int32, _P(MarchParams),
@@ -345,6 +357,24 @@ def reproject(cam: CameraUBO, cam_prev: CameraUBO, source, target, width: int, h
     pp = ctypes.byref(params) if params is not None else None
     check(lib().mm_reproject(ctypes.byref(cam), ctypes.byref(cam_prev), source.data_ptr(), target.data_ptr(), width,
                              height, pitch, pp, stream), "mm_reproject")
+
+
+def shadow_params(**overrides) -> ShadowParams:
+    p = ShadowParams()
+    lib().mm_shadow_params_reference(ctypes.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def cloud_shadow(cam: CameraUBO, sun: SunUBO, sky: SkyUBO, noise: NoiseSet, points, out,
+                 params: ShadowParams | None = None, stream: int | None = None) -> None:
+    """model.frag's shadow march for device points [n, 3] (float32) into out [n]."""
+    stream = current_stream() if stream is None else stream
+    pp = ctypes.byref(params) if params is not None else None
+    check(lib().mm_cloud_shadow(ctypes.byref(cam), ctypes.byref(sun), ctypes.byref(sky), ctypes.byref(noise.weather.desc),
+                                ctypes.byref(noise.low_res.desc), points.data_ptr(), int(points.shape[0]),
+                                out.data_ptr(), pp, stream), "mm_cloud_shadow")
 
 
 def previous_camera(cam: CameraUBO) -> CameraUBO:

@@ -507,6 +507,94 @@ int mm_render_frame(const mm_camera_ubo* camera, const mm_camera_ubo* camera_pre
                       params, nullptr, nullptr, stream);
 }
 
+void mm_shadow_params_reference(mm_shadow_params* p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->size = sizeof(mm_shadow_params);
+    const float R = 1000000.0f;                   // ATMOSPHERE_RADIUS (model.frag:61)
+    p->inner_diameter = R;
+    p->center_y = -R * 0.5f * 0.99f;              // model.frag:247
+    p->thickness = 0.5f * R * 0.02f;              // model.frag:249
+    p->proj_radius = 0.5f * R;                    // model.frag:75
+    p->step = 0.1f * p->thickness;                // model.frag:256
+    p->steps = 6;                                 // NUM_SHADOW_STEPS
+    p->origin_scale = 4.0f;                       // model.frag:260
+    p->wind_strength = 20.0f;                     // WIND_STRENGTH
+    p->weather_scale = 0.00001f;                  // model.frag:110
+    p->low_scale = 0.000057f;                     // model.frag:113
+    p->subtexel_bits = 8;
+}
+
+int mm_cloud_shadow(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                    const mm_texture* weather, const mm_texture* low_res, const float* points, int32_t n, float* out,
+                    const mm_shadow_params* params, void* stream) {
+    if (!camera || !sun || !sky || !weather || !low_res || !points || !out) return MM_E_NULL;
+    if (n < 0) return MM_E_PARAM;
+    mm_shadow_params ref;
+    if (!params) {
+        mm_shadow_params_reference(&ref);
+        params = &ref;
+    }
+    if (params->size != sizeof(mm_shadow_params)) return MM_E_VERSION;
+    const float f[] = {params->inner_diameter, params->center_y, params->thickness, params->proj_radius, params->step,
+                       params->origin_scale, params->wind_strength, params->weather_scale, params->low_scale};
+    if (!finite_n(f, (int)(sizeof(f) / sizeof(f[0])))) return MM_E_PARAM;
+    if (!(params->inner_diameter > 0.f) || !(params->thickness > 0.f) || params->steps < 0 || params->steps > 4096)
+        return MM_E_PARAM;
+    if (params->subtexel_bits != 0 && params->subtexel_bits != 8) return MM_E_PARAM;
+    if (!finite_n(camera->view, 16) || !finite_n(camera->cameraPosition, 4) || !finite_n(sun->directionBasis, 16) ||
+        !finite_n(sky->wind, 4))
+        return MM_E_PARAM;
+    int e;
+    if ((e = check_texture(weather, 0, 2, true))) return e;
+    if ((e = check_texture(low_res, 2, 3, true))) return e;
+    int fset;
+    if (weather->layout == MM_LAYOUT_LINEAR && low_res->layout == MM_LAYOUT_LINEAR && weather->format == MM_FMT_RGBA8 &&
+        low_res->format == MM_FMT_RGBA8)
+        fset = mm::FSET_REF;
+    else if (weather->layout == MM_LAYOUT_APRON && low_res->layout == MM_LAYOUT_APRON &&
+             weather->format == MM_FMT_RG8 && low_res->format == MM_FMT_R8E16)
+        fset = mm::FSET_FAST;
+    else
+        return MM_E_FORMAT;
+    if (n == 0) return MM_OK;
+    mm::ShadowConsts c;
+    memset(&c, 0, sizeof(c));
+    const float* V = camera->view;
+    const float* B = sun->directionBasis;
+    // L = normalize((camera.view * vec4(sunDir, 0)).xyz); flipped below the horizon (model.frag:219-220)
+    float lx = (V[0] * B[4] + V[4] * B[5]) + V[8] * B[6];
+    float ly = (V[1] * B[4] + V[5] * B[5]) + V[9] * B[6];
+    float lz = (V[2] * B[4] + V[6] * B[5]) + V[10] * B[6];
+    const float dd = (lx * lx + ly * ly) + lz * lz;
+    if (!(dd > 0.f)) return MM_E_PARAM;
+    const float r = 1.0f / sqrtf(dd);
+    lx *= r; ly *= r; lz *= r;
+    if (ly < -0.05f) { lx *= -1.0f; ly *= -1.0f; lz *= -1.0f; }
+    c.L[0] = lx; c.L[1] = ly; c.L[2] = lz;
+    for (int k = 0; k < 3; k++) {
+        c.sunW[k] = B[4 + k];
+        c.camPos[k] = camera->cameraPosition[k];
+        c.wind[k] = sky->wind[k];
+    }
+    c.time = sky->wind[3];
+    c.centerY = params->center_y;
+    c.innerDiam = params->inner_diameter;
+    c.thickness = params->thickness;
+    c.projRadius = params->proj_radius;
+    c.step = params->step;
+    c.originScale = params->origin_scale;
+    c.windStrength = params->wind_strength;
+    c.weatherScale = params->weather_scale;
+    c.lowScale = params->low_scale;
+    c.steps = params->steps;
+    c.subtexelBits = params->subtexel_bits;
+    c.fp64 = (params->flags & MM_MARCH_ISECT_FP64) ? 1 : 0;
+    c.n = n;
+    reset_error_state();
+    return mm::launch_shadow(fset, c, view_of(*weather), view_of(*low_res), points, out, stream);
+}
+
 struct mm_view_plan {
     int n_views = 0, total_tiles = 0, grid = 0, fset = 0, mode = 0;
     mm::ViewDesc* d_views = nullptr;  // device: ViewDesc[n_views] then the tile counter

@@ -944,6 +944,76 @@ void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* 
     march_tile<FSET, COUNT, CONE_SURROGATE>(c, tx, tile, out, counts);
 }
 
+// ------------------------------------------------------------------ cloud shadows (model.frag)
+// One work-item per shaded point: the mesh pass's six-step low-res shadow march (model.frag:240-270)
+// with that shader's own constants and cloud functions (model.frag:52-129). Remaps divide exactly
+// like the shader, so the REF texture set reproduces the scalar restatement up to pow's last bit.
+__device__ __forceinline__ float sh_remap(float v, float a, float b, float c, float d) {
+    return c + ((__fdiv_rn(v - a, b - a)) * (d - c));
+}
+__device__ __forceinline__ float sh_remap_clamped(float v, float a, float b, float c, float d) {
+    return clampf(c + ((__fdiv_rn(v - a, b - a)) * (d - c)), c, d);
+}
+__device__ __forceinline__ float sh_layer(float h, float type) {
+    h = clampf(h, 0.0f, 1.0f);
+    const float strato = fmaxf(0.0f, sh_remap(h, 0.0f, 0.2f, 0.0f, 1.0f) * sh_remap(h, 0.2f, 0.7f, 1.0f, 0.0f));
+    const float stratus = fmaxf(0.0f, sh_remap(h, 0.0f, 0.1f, 0.0f, 1.0f) * sh_remap(h, 0.2f, 0.3f, 1.0f, 0.0f));
+    const float d1 = mixf(stratus, strato, clampf(type * 2.0f, 0.0f, 1.0f));
+    const float d2 = mixf(strato, stratus, clampf((type - 0.5f) * 2.0f, 0.0f, 1.0f));
+    return mixf(d1, d2, type);
+}
+
+template <int FSET>
+__device__ float sh_cloud_test(const mm::ShadowConsts& c, const TexView& wt, const TexView& lt, V3 pos, float h,
+                               V3 center) {
+    const V3 proj = (c.projRadius * normalize3(pos - center)) + center;
+    const Weather w = read_weather<FSET>(wt, c.weatherScale * (proj.x - c.camPos[0]),
+                                         c.weatherScale * (proj.z - c.camPos[2]), c.subtexelBits);
+    const float layer = sh_layer(h, w.type);
+    const LowG<FSET> lg = gather_low<FSET>(lt, pos * c.lowScale, c.subtexelBits);
+    float density = layer * sh_remap_clamped(low_shape<FSET>(lg, lt), 0.3f, 1.0f, 0.0f, 1.0f);
+    if (density < 0.0001f) return 0.0f;
+    const float e = clampf(sh_remap(fminf(0.85f, w.coverage), 0.7f, 0.8f, 1.0f, 0.6f), 0.6f, 1.0f);
+    const float coverage = powg(h, e);
+    const float erosion = sh_remap_clamped(low_erosion<FSET>(lg, lt), coverage, 1.0f, 0.0f, 1.0f);
+    return sh_remap_clamped(density, erosion, 1.0f, 0.0f, 1.0f);
+}
+
+template <int FSET>
+__global__ __launch_bounds__(256) void cloud_shadow_kernel(const mm::ShadowConsts c, const TexView wt, const TexView lt,
+                                                           const float* __restrict__ pts, float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= c.n) return;
+    const V3 wc = mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    const V3 sunW = ld3(c.sunW);
+    const V3 camPos = ld3(c.camPos);
+    V3 center = camPos;
+    center.y = c.centerY;
+    bool hit;
+    float t = c.fp64 ? (float)sphere_t<double>(wc.x, wc.y, wc.z, sunW.x, sunW.y, sunW.z, center.x, center.y, center.z,
+                                               (double)c.innerDiam, hit)
+                     : sphere_t<float>(wc.x, wc.y, wc.z, sunW.x, sunW.y, sunW.z, center.x, center.y, center.z,
+                                       c.innerDiam, hit);
+    const V3 L = ld3(c.L);
+    const V3 origin = wc * c.originScale;
+    const V3 wind = ld3(c.wind);
+    float accum = 0.0f;
+    for (int s = 0; s < c.steps; s++) {
+        const V3 pos = origin + L * t;
+        const V3 proj = (c.projRadius * normalize3(pos - center)) + center;
+        const float h = clampf(__fdiv_rn(length3(pos - proj), c.thickness), 0.0f, 1.0f);
+        V3 wo = c.windStrength * (wind + mk(0.0f, 0.2f * h, 0.0f));
+        wo = wo * (c.time + h * 200.0f);
+        accum = fmaxf(sh_cloud_test<FSET>(c, wt, lt, pos + wo, h, center), accum);
+        if (accum > 0.99f) {
+            accum = 1.0f;
+            break;
+        }
+        t += c.step;
+    }
+    out[i] = accum;
+}
+
 // ------------------------------------------------------------------ multi-view persistent kernel
 // A fixed grid of waves (sized to fill every CU) drains one global queue of 64-pixel tiles that
 // spans all views: each wave claims the next tile with one atomic, finds its view by binary search
@@ -1025,6 +1095,19 @@ int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned 
 }  // namespace
 
 namespace mm {
+
+int launch_shadow(int fset, const ShadowConsts& c, const TexView& weather, const TexView& low, const float* pts,
+                  float* out, void* stream) {
+    dim3 grid((unsigned)((c.n + 255) / 256)), block(256);
+    hipStream_t s = (hipStream_t)stream;
+    if (fset == FSET_REF)
+        hipLaunchKernelGGL(cloud_shadow_kernel<FSET_REF>, grid, block, 0, s, c, weather, low, pts, out);
+    else if (fset == FSET_FAST)
+        hipLaunchKernelGGL(cloud_shadow_kernel<FSET_FAST>, grid, block, 0, s, c, weather, low, pts, out);
+    else
+        return MM_E_FORMAT;
+    return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
+}
 
 int views_resident_waves(int fset, int mode, int* out) {
     const void* k = fset == FSET_REF ? views_kernel_ptr_c<FSET_REF>(mode, false)

@@ -77,6 +77,18 @@ struct ReprojConsts {
 };
 int launch_reproject(const ReprojConsts& c, const float* src, float* dst, void* stream);
 
+// Constants of one cloud-shadow pass (model.frag:240-283), derived on the host.
+struct ShadowConsts {
+    float L[3];                   // view-space sun direction, flipped when L.y < -0.05
+    float sunW[3];                // sun.directionBasis[1].xyz (world, not normalised)
+    float camPos[3];
+    float wind[3], time;
+    float centerY, innerDiam, thickness, projRadius, step, originScale, windStrength, weatherScale, lowScale;
+    int steps, subtexelBits, fp64, n;
+};
+int launch_shadow(int fset, const ShadowConsts& c, const TexView& weather, const TexView& low, const float* pts,
+                  float* out, void* stream);
+
 // One view of a multi-view launch (device array, built on the host).
 struct ViewDesc {
     MarchConsts c;
