@@ -187,7 +187,7 @@ typedef struct mm_march_params {
     int32_t subtexel_bits;      /* sampler weight precision: 8 = fixed-point like the hardware, 0 = float */
     /* light cone */
     int32_t cone_mode;          /* MM_CONE_* */
-    float surrogate_threshold;  /* jobs whose predicted error bound is >= this take the plain cone */
+    float surrogate_threshold;  /* jobs whose weighted error bound is >= this take the plain cone */
     const void* surrogate;      /* device model block of mm_surrogate_bytes() (MM_CONE_SURROGATE);
                                    REF and FAST texture sets only */
     int32_t tile_width;         /* pixels per wave tile row: 8 (8x8, default when 0), 16 (16x4), 32 (32x2);
@@ -315,8 +315,10 @@ void mm_view_plan_destroy(mm_view_plan* plan);
 /* ---------------------------------------------------------------- learned light-cone surrogate
  * No counterpart in the reference: an optional replacement of the six-sample light cone
  * (comp:440-453) by a 16-64-64-64-2 SiLU MLP evaluated with BF16 MFMA inside the march. For each
- * cone job it predicts the density along the light D and an error bound e (transmittance units);
- * jobs with e >= params.surrogate_threshold take the plain cone, so threshold 0 is the plain march.
+ * cone job it predicts the density along the light D and an error bound e on exp(-D); the job's
+ * result enters the pixel's transmittance with weight (1 - accum) * inScatter * HG (comp:464), and
+ * jobs whose weighted bound e * (1 - accum) * |inScatter * HG| is >= params.surrogate_threshold
+ * take the plain cone, so threshold 0 is the plain march.
  * A model is one device block of mm_surrogate_bytes() bytes (FP32 parameters, Adam state, feature
  * statistics, packed BF16 fragments). Training samples are rows of MM_SURROGATE_ROW floats:
  * 16 features followed by the cone's D, recorded by mm_march_clouds_collect. */

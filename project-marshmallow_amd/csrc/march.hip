@@ -871,7 +871,7 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
                 }
             }
         } else {
-            // surrogate: features + MLP for every queued job, plain cone only where the predicted
+            // surrogate: features + MLP for every queued job, plain cone only where the job's weighted
             // error bound is not below the threshold (compacted onto the first lanes)
             float f[surr::kIn];
             for (int k = 0; k < surr::kIn; k++) f[k] = 0.0f;
@@ -880,7 +880,9 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
                                            q.lo[lane], q.dens[lane], f, nd);
             for (int k = 0; k < surr::kIn; k++) f[k] = (f[k] - c.surrStats[k]) * c.surrStats[16 + k];
             const surr::FwdOut o = surr::mlp_forward64(c.surrFrags, c.surrParams, f);
-            const bool fb = lane < nb && !(surr::softplus(o.o1) < c.surrThreshold);
+            // the job's share of the pixel: its cone result enters T with weight (1 - accum) * inScatter * HG
+            const float bound = lane < nb ? surr::softplus(o.o1) * (q.w[lane] * fabsf(q.x0[lane])) : 0.0f;
+            const bool fb = lane < nb && !(bound < c.surrThreshold);
             const unsigned long long fm = __ballot(fb);
             const int nf = __popcll(fm);
             if (fb) q.fallback[__builtin_amdgcn_mbcnt_hi((unsigned)(fm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)fm, 0u))] = lane;
