@@ -113,6 +113,29 @@ __device__ __forceinline__ typename TexelT<BYTES>::type ld_texel(const uint8_t* 
     else return ld_u64(p);
 }
 
+// Two x-adjacent texels in one load (apron layouts: texel x0 + 1 is always stored after x0). The
+// pair is only aligned to the texel size; gfx950 serves such under-aligned global loads.
+typedef uint32_t __attribute__((aligned(2))) u32_a2;
+struct alignas(4) u64_a4 { uint32_t x, y; };
+struct alignas(8) u128_a8 { uint32_t x, y, z, w; };
+template <int BYTES>
+__device__ __forceinline__ void ld_pair(const uint8_t* p, typename TexelT<BYTES>::type& a,
+                                        typename TexelT<BYTES>::type& b) {
+    if constexpr (BYTES == 2) {
+        const uint32_t v = *reinterpret_cast<const u32_a2*>(p);
+        a = v & 0xffffu;
+        b = v >> 16;
+    } else if constexpr (BYTES == 4) {
+        const u64_a4 v = *reinterpret_cast<const u64_a4*>(p);
+        a = v.x;
+        b = v.y;
+    } else {
+        const u128_a8 v = *reinterpret_cast<const u128_a8*>(p);
+        a = make_uint2(v.x, v.y);
+        b = make_uint2(v.z, v.w);
+    }
+}
+
 template <int BYTES> struct G2 {
     typename TexelT<BYTES>::type t[4];
     float fx, fy;
@@ -135,10 +158,15 @@ __device__ __forceinline__ G2<BYTES> gather2(const TexView& tv, float u, float v
     else { x1 = (x0 + 1) & (tv.w - 1); y1 = (y0 + 1) & (tv.h - 1); }
     const uint8_t* r0 = tv.data + (size_t)(y0 * tv.stride) * BYTES;
     const uint8_t* r1 = tv.data + (size_t)(y1 * tv.stride) * BYTES;
-    g.t[0] = ld_texel<BYTES>(r0 + x0 * BYTES);
-    g.t[1] = ld_texel<BYTES>(r0 + x1 * BYTES);
-    g.t[2] = ld_texel<BYTES>(r1 + x0 * BYTES);
-    g.t[3] = ld_texel<BYTES>(r1 + x1 * BYTES);
+    if constexpr (APRON) {
+        ld_pair<BYTES>(r0 + x0 * BYTES, g.t[0], g.t[1]);
+        ld_pair<BYTES>(r1 + x0 * BYTES, g.t[2], g.t[3]);
+    } else {
+        g.t[0] = ld_texel<BYTES>(r0 + x0 * BYTES);
+        g.t[1] = ld_texel<BYTES>(r0 + x1 * BYTES);
+        g.t[2] = ld_texel<BYTES>(r1 + x0 * BYTES);
+        g.t[3] = ld_texel<BYTES>(r1 + x1 * BYTES);
+    }
     return g;
 }
 
@@ -158,14 +186,21 @@ __device__ __forceinline__ G3<BYTES> gather3(const TexView& tv, float u, float v
     const size_t r10 = (size_t)(z0 * tv.sliceStride + y1 * tv.stride);
     const size_t r01 = (size_t)(z1 * tv.sliceStride + y0 * tv.stride);
     const size_t r11 = (size_t)(z1 * tv.sliceStride + y1 * tv.stride);
-    g.t[0] = ld_texel<BYTES>(b + (r00 + x0) * BYTES);
-    g.t[1] = ld_texel<BYTES>(b + (r00 + x1) * BYTES);
-    g.t[2] = ld_texel<BYTES>(b + (r10 + x0) * BYTES);
-    g.t[3] = ld_texel<BYTES>(b + (r10 + x1) * BYTES);
-    g.t[4] = ld_texel<BYTES>(b + (r01 + x0) * BYTES);
-    g.t[5] = ld_texel<BYTES>(b + (r01 + x1) * BYTES);
-    g.t[6] = ld_texel<BYTES>(b + (r11 + x0) * BYTES);
-    g.t[7] = ld_texel<BYTES>(b + (r11 + x1) * BYTES);
+    if constexpr (APRON) {
+        ld_pair<BYTES>(b + (r00 + x0) * BYTES, g.t[0], g.t[1]);
+        ld_pair<BYTES>(b + (r10 + x0) * BYTES, g.t[2], g.t[3]);
+        ld_pair<BYTES>(b + (r01 + x0) * BYTES, g.t[4], g.t[5]);
+        ld_pair<BYTES>(b + (r11 + x0) * BYTES, g.t[6], g.t[7]);
+    } else {
+        g.t[0] = ld_texel<BYTES>(b + (r00 + x0) * BYTES);
+        g.t[1] = ld_texel<BYTES>(b + (r00 + x1) * BYTES);
+        g.t[2] = ld_texel<BYTES>(b + (r10 + x0) * BYTES);
+        g.t[3] = ld_texel<BYTES>(b + (r10 + x1) * BYTES);
+        g.t[4] = ld_texel<BYTES>(b + (r01 + x0) * BYTES);
+        g.t[5] = ld_texel<BYTES>(b + (r01 + x1) * BYTES);
+        g.t[6] = ld_texel<BYTES>(b + (r11 + x0) * BYTES);
+        g.t[7] = ld_texel<BYTES>(b + (r11 + x1) * BYTES);
+    }
     return g;
 }
 
