@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--fset", default="fast", choices=["fast", "ref", "prec"])
     ap.add_argument("--no-batch", action="store_true", help="evaluate light cones in place (lock-step kernel)")
+    ap.add_argument("--tile-width", type=int, default=0, help="8, 16 or 32 (0 = library default)")
+    ap.add_argument("--tiles-per-wave", type=int, default=0, help="tiles per wave with ray refill (0 = default)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device-to-device copies of MIB MiB (HBM counter calibration)")
@@ -126,6 +128,8 @@ def main():
     prm = mm.march_params("scenario")
     if args.no_batch:
         prm.flags |= mm.MARCH_NO_BATCH
+    prm.tile_width = args.tile_width
+    prm.tiles_per_wave = args.tiles_per_wave
     ns = mm.NoiseSet(args.fset)
     nparams = mm.noise_params(seed=1, weather_seed=1, coverage=0.5)
     out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
@@ -166,7 +170,7 @@ def main():
     achieved = alg_bytes / (march_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(args.fset, W, H),
-                "kernel": "march_kernel", "kernel_ms": round(march_ms, 4),
+                "kernel": "march_kernel" if args.no_batch else "march_batched_kernel", "kernel_ms": round(march_ms, 4),
                 "algorithmic_bytes_per_launch": alg_bytes,
                 "bytes_per_density_sample": b["density"], "bytes_per_detail_sample": b["detail"]}
     rec = {
@@ -186,13 +190,15 @@ def main():
                                f"(Earth radius 6.371e6 m, shell 1500-4000 m, camera 200 m pitched +20 deg, 60 deg "
                                f"FOV, sun 25 deg, coverage 0.5, t = 0, weather seed 1)",
                    "width": W, "height": H, "format_set": args.fset,
-                   "kernel": "lock-step" if args.no_batch else "cone-batched", "coarse_steps_max": 128, "fine_steps_max": 256},
+                   "kernel": "lock-step" if args.no_batch else "cone-batched", "coarse_steps_max": 128, "fine_steps_max": 256,
+                   "tile_width": args.tile_width or 8, "tiles_per_wave": args.tiles_per_wave or 1},
         "ms_per_frame": round(ms_step, 4),
         "frames_per_s": round(1000.0 / ms_step, 2),
         "march_ms": round(march_ms, 4),
         "noise_ms": round(noise_ms, 4),
         "density_samples_per_frame": n_density,
         "detail_samples_per_frame": n_detail,
+        "cone_jobs_per_frame": n_jobs,
         "wall_s": round(wall, 3),
         "roofline": roofline,
         "cpu_baseline": None,

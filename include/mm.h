@@ -192,7 +192,9 @@ typedef struct mm_march_params {
                                    REF and FAST texture sets only */
     int32_t tile_width;         /* pixels per wave tile row: 8 (8x8, default when 0), 16 (16x4), 32 (32x2);
                                    the image does not depend on it */
-    int32_t _reserved[7];
+    int32_t tiles_per_wave;     /* tiles one wave marches, refilling lanes whose ray has ended with the
+                                   next pixel (0 = default); the image does not depend on it */
+    int32_t _reserved[6];
 } mm_march_params;
 
 /* Preset with the reference's constants (compute-clouds.comp, bit-for-bit). */

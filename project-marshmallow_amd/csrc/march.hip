@@ -827,31 +827,171 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     return x;
 }
 
+// Evaluates the first min(qn, 64) queued cone jobs, one per lane, leaving each job's term
+// x0 * Beer(D) in q.res (and, in the recorder mode, appending training rows).
 template <int FSET, bool COUNT, int MODE>
-__device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchTextures& tx, const int tile,
-                                           float* __restrict__ out, unsigned long long* __restrict__ counts) {
+__device__ __forceinline__ void run_batch(const MarchConsts& c, const MarchTextures& tx, ConeQueue& q, const int nb,
+                                          int& nd, int& nh, int& nj, int& nc) {
+    const int lane = threadIdx.x;
+    if (COUNT) nj += nb;
+    if constexpr (MODE == CONE_PLAIN) {
+        if (COUNT) nc += nb;
+        if (lane < nb) {
+            float dal = cone_dal<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], nd, nh);
+            q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
+        }
+    } else if constexpr (MODE == CONE_COLLECT) {
+        if (COUNT) nc += nb;
+        // plain cone + record (features, D) of one job in c.collectKeep
+        float f[surr::kIn];
+        float dal = 0.0f;
+        bool keep = false;
+        if (lane < nb) {
+            const V3 pos = mk(q.x[lane], q.y[lane], q.z[lane]);
+            dal = cone_dal<FSET, COUNT>(c, tx, pos, q.step[lane], nd, nh);
+            q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
+            const uint32_t hsh = mix32(__float_as_uint(pos.x) ^ mix32(__float_as_uint(pos.y) ^
+                                       mix32(__float_as_uint(pos.z) ^ c.collectSeed)));
+            keep = (hsh % c.collectKeep) == 0u;
+            if (keep) cone_features<FSET, COUNT>(c, tx, pos, q.step[lane], q.rh[lane], q.lo[lane], q.dens[lane], f, nd);
+        }
+        const unsigned long long km = __ballot(keep);
+        if (km) {
+            unsigned base = 0;
+            if (lane == 0) base = atomicAdd(c.collectCount, (unsigned)__popcll(km));
+            base = __shfl(base, 0);
+            const unsigned slot = base + __builtin_amdgcn_mbcnt_hi((unsigned)(km >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
+            if (keep && slot < c.collectCap) {
+                float* dst = c.collectBuf + (size_t)slot * (surr::kIn + 1);
+                for (int k = 0; k < surr::kIn; k++) dst[k] = f[k];
+                dst[surr::kIn] = dal;
+            }
+        }
+    } else {
+        // surrogate: features + MLP for every queued job, plain cone only where the job's weighted
+        // error bound is not below the threshold (compacted onto the first lanes)
+        float f[surr::kIn];
+        for (int k = 0; k < surr::kIn; k++) f[k] = 0.0f;
+        if (lane < nb)
+            cone_features<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], q.rh[lane],
+                                       q.lo[lane], q.dens[lane], f, nd);
+        for (int k = 0; k < surr::kIn; k++) f[k] = (f[k] - c.surrStats[k]) * c.surrStats[16 + k];
+        const surr::FwdOut o = surr::mlp_forward64(c.surrFrags, c.surrParams, f);
+        // the job's share of the pixel: its cone result enters T with weight (1 - accum) * inScatter * HG
+        const float bound = lane < nb ? surr::softplus(o.o1) * (q.w[lane] * fabsf(q.x0[lane])) : 0.0f;
+        const bool fb = lane < nb && !(bound < c.surrThreshold);
+        const unsigned long long fm = __ballot(fb);
+        const int nf = __popcll(fm);
+        if (fb) q.fallback[__builtin_amdgcn_mbcnt_hi((unsigned)(fm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)fm, 0u))] = lane;
+        __syncthreads();
+        if (COUNT) nc += nf;
+        if (lane < nf) {
+            const int j = q.fallback[lane];
+            q.dal[j] = cone_dal<FSET, COUNT>(c, tx, mk(q.x[j], q.y[j], q.z[j]), q.step[j], nd, nh);
+        }
+        __syncthreads();
+        if (lane < nb) {
+            const float dal = fb ? q.dal[lane] : fmaxf(0.0f, -o.o0);
+            q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
+        }
+    }
+    __syncthreads();
+}
+
+// Drops the first nb (= 64 when qn > 64) jobs of the queue, moving the remainder to the front.
+__device__ __forceinline__ void queue_shift(ConeQueue& q, const int nb, const int rem) {
+    const int lane = threadIdx.x;
+    __syncthreads();
+    if (lane < rem) {  // rem < 64 and only when nb == 64: source and destination slots are disjoint
+        const int s = nb + lane;
+        q.x[lane] = q.x[s]; q.y[lane] = q.y[s]; q.z[lane] = q.z[s]; q.step[lane] = q.step[s];
+        q.x0[lane] = q.x0[s]; q.w[lane] = q.w[s]; q.blend[lane] = q.blend[s]; q.owner[lane] = q.owner[s];
+        q.rh[lane] = q.rh[s]; q.lo[lane] = q.lo[s]; q.dens[lane] = q.dens[s];
+    }
+    __syncthreads();
+}
+
+// Constant-address-space pointer: reads through it are scalar loads (uniform, scalar-cached).
+typedef const __attribute__((address_space(4))) MarchConsts* ConstsPtr;
+
+template <class T>
+__device__ __forceinline__ ConstsPtr consts_ptr(const T* p) {
+    return (ConstsPtr)(p);
+}
+
+constexpr int kRefillMin = 16;  // free lanes that interrupt the primary march to take new pixels
+
+// One wave marches the 64 * ntiles pixels of tiles [tile0, tile0 + ntiles) (of the marched grid,
+// (1 << tileShift) x (64 >> tileShift) pixels each, lane-major in x). Lanes whose ray has finished
+// take the next pixel of the range (ballot + mbcnt), so the primary march keeps the wave busy
+// across ray lengths; a pixel's output is written once its ray has ended and all its queued cone
+// jobs are folded in. Every pixel sees exactly the shader's operations in the shader's order, so
+// the image does not depend on ntiles or the tile shape.
+template <int FSET, bool COUNT, int MODE>
+__device__ __forceinline__ void march_tiles(ConstsPtr c0, const MarchTextures& tx, const int tile0,
+                                            const int ntiles, float* __restrict__ out,
+                                            unsigned long long* __restrict__ counts) {
     __shared__ ConeQueue q;
     const int lane = threadIdx.x;
-    // tile of (1 << tileShift) x (64 >> tileShift) pixels of the marched grid, lane-major in x
-    const int gi = (tile % c.tilesX) * (1 << c.tileShift) + (lane & ((1 << c.tileShift) - 1));
-    const int gj = (tile / c.tilesX) * (64 >> c.tileShift) + (lane >> c.tileShift);
-    const bool valid = gi < c.gridW && gj < c.gridH;
-    int px = 0, py = 0;
-    if (valid) tile_pixel(c, gi, gj, px, py);
+    const int tw = 1 << ((const MarchConsts*)c0)->tileShift, th = 64 >> ((const MarchConsts*)c0)->tileShift;
+    const int total = ntiles * 64;  // pixel slots of this wave
+    int next = 0;                   // next pixel slot (wave-uniform)
     int nd = 0, nh = 0, nj = 0, nc = 0;  // density, detail samples; cone jobs, jobs on the plain cone
     RaySetup r;
     MarchState m;
     m.active = false;
-    if (valid) {
-        r = ray_setup(c, tx, px, py);
-        if (!r.march)
-            reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = make_float4(r.bg.x, r.bg.y, r.bg.z, r.alpha);
-        march_init(c, r, m);
-    }
+    bool has = false;  // the lane holds a pixel whose output is not written yet
+    int px = 0, py = 0, pend = 0;
     float transmittance = 1.0f;
     int qn = 0;  // queued jobs (wave-uniform)
     for (;;) {
-        // primary march until a full batch is queued or every ray has left its loop
+        // The constants are re-read (scalar loads) every iteration through a pointer the optimiser
+        // cannot follow: hoisted out of the loop, the set-up constants of a refill and the march
+        // constants would all stay live in scalar registers at once and spill into vector ones.
+        ConstsPtr cp = c0;
+        {
+            const uint64_t v = (uint64_t)cp;
+            cp = (ConstsPtr)((((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32))) << 32) |
+                             __builtin_amdgcn_readfirstlane((uint32_t)v));
+        }
+        asm volatile("" : "+s"(cp));
+        const MarchConsts& c = *(const MarchConsts*)cp;
+        // a pixel whose ray has ended and whose cone jobs are all folded in is complete
+        if (has && !m.active && pend == 0) {
+            reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
+            has = false;
+        }
+        // refill lanes without a pixel
+        const unsigned long long need = __ballot(!has);
+        if (need != 0ull && next < total) {
+            const int k = next + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(need >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((unsigned)need, 0u));
+            next += __popcll(need);
+            if (!has && k < total) {
+                const int tile = tile0 + (k >> 6), li = k & 63;
+                const int gi = (tile % c.tilesX) * tw + (li & (tw - 1)), gj = (tile / c.tilesX) * th + (li >> c.tileShift);
+                if (gi < c.gridW && gj < c.gridH) {
+                    tile_pixel(c, gi, gj, px, py);
+                    r = ray_setup(c, tx, px, py);
+                    if (!r.march) {
+                        reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] =
+                            make_float4(r.bg.x, r.bg.y, r.bg.z, r.alpha);
+                    } else {
+                        march_init(c, r, m);
+                        transmittance = 1.0f;
+                        pend = 0;
+                        has = true;
+                    }
+                }
+            }
+        }
+        if (!__any(has)) {  // then the queue is empty too (jobs belong to unwritten pixels)
+            if (next >= total) break;
+            continue;
+        }
+        // primary march until a full batch is queued, every ray has left its loop, or enough lanes
+        // are free to take new pixels
         while (qn < 64 && __any(m.active)) {
             V3 jp;
             float js = 0.f, jx0 = 0.f, jw = 0.f, jrh = 0.f, jlo = 0.f, jd = 0.f;
@@ -864,90 +1004,28 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
                 q.x[slot] = jp.x; q.y[slot] = jp.y; q.z[slot] = jp.z; q.step[slot] = js;
                 q.x0[slot] = jx0; q.w[slot] = jw; q.blend[slot] = r.beerBlend; q.owner[slot] = lane;
                 q.rh[slot] = jrh; q.lo[slot] = jlo; q.dens[slot] = jd;
+                pend++;
             }
             qn += __popcll(mask);
+            if (next < total && __popcll(__ballot(!has || (!m.active && pend == 0))) >= kRefillMin) break;
         }
-        if (qn == 0) break;
-        __syncthreads();
-        const int nb = qn < 64 ? qn : 64;
-        if (COUNT) nj += nb;
-        if constexpr (MODE == CONE_PLAIN) {
-            if (COUNT) nc += nb;
-            if (lane < nb) {
-                float dal = cone_dal<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], nd, nh);
-                q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
-            }
-        } else if constexpr (MODE == CONE_COLLECT) {
-            if (COUNT) nc += nb;
-            // plain cone + record (features, D) of one job in c.collectKeep
-            float f[surr::kIn];
-            float dal = 0.0f;
-            bool keep = false;
-            if (lane < nb) {
-                const V3 pos = mk(q.x[lane], q.y[lane], q.z[lane]);
-                dal = cone_dal<FSET, COUNT>(c, tx, pos, q.step[lane], nd, nh);
-                q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
-                const uint32_t hsh = mix32(__float_as_uint(pos.x) ^ mix32(__float_as_uint(pos.y) ^
-                                           mix32(__float_as_uint(pos.z) ^ c.collectSeed)));
-                keep = (hsh % c.collectKeep) == 0u;
-                if (keep) cone_features<FSET, COUNT>(c, tx, pos, q.step[lane], q.rh[lane], q.lo[lane], q.dens[lane], f, nd);
-            }
-            const unsigned long long km = __ballot(keep);
-            if (km) {
-                unsigned base = 0;
-                if (lane == 0) base = atomicAdd(c.collectCount, (unsigned)__popcll(km));
-                base = __shfl(base, 0);
-                const unsigned slot = base + __builtin_amdgcn_mbcnt_hi((unsigned)(km >> 32),
-                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
-                if (keep && slot < c.collectCap) {
-                    float* dst = c.collectBuf + (size_t)slot * (surr::kIn + 1);
-                    for (int k = 0; k < surr::kIn; k++) dst[k] = f[k];
-                    dst[surr::kIn] = dal;
+        const bool canRefill = next < total && __any(!has || (!m.active && pend == 0));
+        // a batch runs when full; a partial one when nothing else can move, or when half the wave
+        // waits on queued cone jobs to complete its pixels
+        if (qn >= 64 || (qn > 0 && ((!__any(m.active) && !canRefill) ||
+                                    __popcll(__ballot(has && !m.active)) >= 32))) {
+            __syncthreads();
+            const int nb = qn < 64 ? qn : 64;
+            run_batch<FSET, COUNT, MODE>(c, tx, q, nb, nd, nh, nj, nc);
+            for (int s = 0; s < nb; s++)
+                if (q.owner[s] == lane) {
+                    transmittance = mixf(transmittance, q.res[s], q.w[s]);
+                    pend--;
                 }
-            }
-        } else {
-            // surrogate: features + MLP for every queued job, plain cone only where the job's weighted
-            // error bound is not below the threshold (compacted onto the first lanes)
-            float f[surr::kIn];
-            for (int k = 0; k < surr::kIn; k++) f[k] = 0.0f;
-            if (lane < nb)
-                cone_features<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], q.rh[lane],
-                                           q.lo[lane], q.dens[lane], f, nd);
-            for (int k = 0; k < surr::kIn; k++) f[k] = (f[k] - c.surrStats[k]) * c.surrStats[16 + k];
-            const surr::FwdOut o = surr::mlp_forward64(c.surrFrags, c.surrParams, f);
-            // the job's share of the pixel: its cone result enters T with weight (1 - accum) * inScatter * HG
-            const float bound = lane < nb ? surr::softplus(o.o1) * (q.w[lane] * fabsf(q.x0[lane])) : 0.0f;
-            const bool fb = lane < nb && !(bound < c.surrThreshold);
-            const unsigned long long fm = __ballot(fb);
-            const int nf = __popcll(fm);
-            if (fb) q.fallback[__builtin_amdgcn_mbcnt_hi((unsigned)(fm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)fm, 0u))] = lane;
-            __syncthreads();
-            if (COUNT) nc += nf;
-            if (lane < nf) {
-                const int j = q.fallback[lane];
-                q.dal[j] = cone_dal<FSET, COUNT>(c, tx, mk(q.x[j], q.y[j], q.z[j]), q.step[j], nd, nh);
-            }
-            __syncthreads();
-            if (lane < nb) {
-                const float dal = fb ? q.dal[lane] : fmaxf(0.0f, -o.o0);
-                q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
-            }
+            queue_shift(q, nb, qn - nb);
+            qn -= nb;
         }
-        __syncthreads();
-        for (int s = 0; s < nb; s++)
-            if (q.owner[s] == lane) transmittance = mixf(transmittance, q.res[s], q.w[s]);
-        const int rem = qn - nb;  // < 64 and only when nb == 64: source and destination slots are disjoint
-        __syncthreads();
-        if (lane < rem) {
-            const int s = nb + lane;
-            q.x[lane] = q.x[s]; q.y[lane] = q.y[s]; q.z[lane] = q.z[s]; q.step[lane] = q.step[s];
-            q.x0[lane] = q.x0[s]; q.w[lane] = q.w[s]; q.blend[lane] = q.blend[s]; q.owner[lane] = q.owner[s];
-            q.rh[lane] = q.rh[s]; q.lo[lane] = q.lo[s]; q.dens[lane] = q.dens[s];
-        }
-        __syncthreads();
-        qn = rem;
     }
-    if (valid && r.march) reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
     if (COUNT) {
         atomicAdd(&counts[0], (unsigned long long)nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
@@ -959,12 +1037,14 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
 }
 
 template <int FSET, bool COUNT, int MODE>
-__global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, const MarchTextures tx,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void march_batched_kernel(const MarchConsts c, const MarchTextures tx,
                                                            float* __restrict__ out,
                                                            unsigned long long* __restrict__ counts) {
-    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
-    if (tile >= c.tilesX * c.tilesY) return;  // uniform across the block
-    march_tile<FSET, COUNT, MODE>(c, tx, tile, out, counts);
+    const int g = xcd_tile(blockIdx.x, c.groupsPerXcd);
+    const int tile0 = g * c.tilesPerWave, ntiles = min(c.tilesPerWave, c.tilesX * c.tilesY - tile0);
+    if (ntiles <= 0) return;  // uniform across the block
+    // c is the first kernel argument: it sits at offset 0 of the kernel-argument segment
+    march_tiles<FSET, COUNT, MODE>(consts_ptr(__builtin_amdgcn_kernarg_segment_ptr()), tx, tile0, ntiles, out, counts);
 }
 
 #ifndef MM_SURROGATE_WAVES_PER_EU
@@ -976,9 +1056,11 @@ template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
 void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
                             unsigned long long* __restrict__ counts) {
-    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
-    if (tile >= c.tilesX * c.tilesY) return;
-    march_tile<FSET, COUNT, CONE_SURROGATE>(c, tx, tile, out, counts);
+    const int g = xcd_tile(blockIdx.x, c.groupsPerXcd);
+    const int tile0 = g * c.tilesPerWave, ntiles = min(c.tilesPerWave, c.tilesX * c.tilesY - tile0);
+    if (ntiles <= 0) return;
+    march_tiles<FSET, COUNT, CONE_SURROGATE>(consts_ptr(__builtin_amdgcn_kernarg_segment_ptr()), tx, tile0, ntiles, out,
+                                             counts);
 }
 
 // ------------------------------------------------------------------ cloud shadows (model.frag)
@@ -1079,7 +1161,10 @@ __device__ __forceinline__ void views_loop(const ViewDesc* __restrict__ views, i
         }
         const ViewDesc& v = views[lo];
         tx.weather = v.weather;
-        march_tile<FSET, COUNT, MODE>(v.c, tx, (int)t - v.tileBase, v.out, counts);
+        const int g = (int)t - v.tileBase;  // group of tilesPerWave tiles of this view
+        const int tile0 = g * v.c.tilesPerWave;
+        march_tiles<FSET, COUNT, MODE>(consts_ptr(&v.c), tx, tile0, min(v.c.tilesPerWave, v.c.tilesX * v.c.tilesY - tile0), v.out,
+                                       counts);
     }
 }
 
@@ -1113,7 +1198,7 @@ const void* views_kernel_ptr_c(int mode, bool count) {
 
 template <int FSET, bool COUNT>
 int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts, hipStream_t s) {
-    dim3 grid((unsigned)(8 * c.tilesPerXcd)), block(64);
+    dim3 grid((unsigned)(8 * ((c.flags & MM_MARCH_NO_BATCH) ? c.tilesPerXcd : c.groupsPerXcd))), block(64);
     if (c.coneMode == CONE_PLAIN && (c.flags & MM_MARCH_NO_BATCH))
         hipLaunchKernelGGL((march_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
     else if (c.coneMode == CONE_PLAIN)

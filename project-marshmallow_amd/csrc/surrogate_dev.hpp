@@ -102,6 +102,15 @@ __device__ __forceinline__ FwdOut mlp_forward_tile(const uint4* __restrict__ fra
     return r;
 }
 
+// A wave-uniform pointer made explicit (scalar registers), whatever the optimiser could prove.
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T*)(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 // Bias of rows crow(r, h) of m-tile `off` as an accumulator initialiser.
@@ -159,6 +168,8 @@ __device__ __forceinline__ FwdOut mlp_forward64(const uint4* frags, const float*
     const int lane = threadIdx.x & 63, h = lane >> 5;
     // opaque to loop-invariant code motion: keeps the weight loads from being hoisted out of an
     // enclosing loop (where they would hold registers for the whole loop)
+    frags = uniform_ptr(frags);
+    params = uniform_ptr(params);
     asm volatile("" : "+s"(frags));
     asm volatile("" : "+s"(params));
     const bf16x8 lo = pack8(x), hi = pack8(x + 8);
