@@ -283,12 +283,13 @@ int mm_cloud_shadow(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm
 
 /* ---------------------------------------------------------------- many views in one launch
  * No counterpart in the reference (which dispatches one view per frame, VulkanApplication.cpp:
- * 1061-1071): many views of any resolution marched by one persistent launch over a global queue of
- * 64-pixel tiles. Views share the curl and noise volumes and bring their own uniforms, weather map
+ * 1061-1071): many views of any resolution marched by one launch over the 64-pixel tiles of all
+ * views (grouped params.tiles_per_wave at a time), handed to the CUs by the hardware dispatcher as
+ * slots free up. Views share the curl and noise volumes and bring their own uniforms, weather map
  * and output image. Each view's image is identical to mm_march_clouds of that view with the same
- * params. A plan holds the per-view constants in device memory; mm_march_views enqueues only
- * kernels (no allocation, no host transfer, no memset), so it can be captured into a HIP graph and
- * replayed; calls on one plan must not overlap (they share its tile counter). */
+ * params. A plan holds the per-view constants in device memory; mm_march_views enqueues one kernel
+ * (no allocation, no host transfer, no memset), so it can be captured into a HIP graph and
+ * replayed. */
 typedef struct mm_view {
     mm_camera_ubo camera;
     mm_sun_ubo sun;
@@ -308,7 +309,7 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
 /* Replace the views' uniforms / outputs (same count) without reallocating. Synchronises the stream. */
 int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_views, const mm_march_params* params,
                         void* stream);
-/* Total tiles and the persistent grid (waves) of the plan. */
+/* Work items (groups of tiles_per_wave tiles) and the launch's grid (waves) of the plan. */
 int mm_view_plan_info(const mm_view_plan* plan, int32_t* total_tiles, int32_t* grid_waves);
 /* counts: NULL or device uint64[4] accumulators as for mm_march_clouds_counted. */
 int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* stream);

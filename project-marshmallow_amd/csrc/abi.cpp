@@ -534,8 +534,9 @@ void mm_shadow_params_reference(mm_shadow_params* p) {
 int mm_cloud_shadow(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
                     const mm_texture* weather, const mm_texture* low_res, const float* points, int32_t n, float* out,
                     const mm_shadow_params* params, void* stream) {
-    if (!camera || !sun || !sky || !weather || !low_res || !points || !out) return MM_E_NULL;
+    if (!camera || !sun || !sky || !weather || !low_res) return MM_E_NULL;
     if (n < 0) return MM_E_PARAM;
+    if (n > 0 && (!points || !out)) return MM_E_NULL;  // an empty batch may pass NULL buffers
     mm_shadow_params ref;
     if (!params) {
         mm_shadow_params_reference(&ref);
@@ -602,9 +603,8 @@ int mm_cloud_shadow(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm
 }
 
 struct mm_view_plan {
-    int n_views = 0, total_tiles = 0, grid = 0, fset = 0, mode = 0;
-    mm::ViewDesc* d_views = nullptr;  // device: ViewDesc[n_views] then the tile counter
-    unsigned* d_counter = nullptr;
+    int n_views = 0, total_tiles = 0, fset = 0, mode = 0;
+    mm::ViewDesc* d_views = nullptr;  // device: ViewDesc[n_views]
     mm::MarchTextures shared;
     mm_noise_set shared_set;          // the shared textures as given (curl, low_res, hi_res)
     int device = 0;
@@ -667,16 +667,15 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
     p->shared = tx;
     p->shared_set = *shared;
     (void)hipGetDevice(&p->device);
-    if ((e = mm::views_resident_waves(fset, p->mode, &p->grid))) {
+    if (fset == mm::FSET_PREC && p->mode == MM_CONE_SURROGATE) {
         delete p;
-        return e;
+        return MM_E_FORMAT;
     }
     const size_t bytes = sizeof(mm::ViewDesc) * (size_t)n_views;
-    if (hipMalloc((void**)&p->d_views, bytes + 256) != hipSuccess) {
+    if (hipMalloc((void**)&p->d_views, bytes) != hipSuccess) {
         delete p;
         return MM_E_DEVICE;
     }
-    p->d_counter = (unsigned*)((uint8_t*)p->d_views + ((bytes + 255) / 256) * 256);
     hipStream_t s = (hipStream_t)stream;
     if (hipMemcpyAsync(p->d_views, h.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -718,7 +717,7 @@ int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_view
 int mm_view_plan_info(const mm_view_plan* plan, int32_t* total_tiles, int32_t* grid_waves) {
     if (!plan) return MM_E_NULL;
     if (total_tiles) *total_tiles = plan->total_tiles;
-    if (grid_waves) *grid_waves = plan->grid;
+    if (grid_waves) *grid_waves = 8 * ((plan->total_tiles + 7) / 8);
     return MM_OK;
 }
 
@@ -726,8 +725,8 @@ int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* s
     if (!plan) return MM_E_NULL;
     if (plan->total_tiles == 0) return MM_OK;
     reset_error_state();
-    return mm::launch_views(plan->fset, plan->mode, plan->d_views, plan->n_views, plan->shared, plan->d_counter,
-                            plan->total_tiles, plan->grid, counts, stream);
+    return mm::launch_views(plan->fset, plan->mode, plan->d_views, plan->n_views, plan->shared, plan->total_tiles,
+                            counts, stream);
 }
 
 void mm_view_plan_destroy(mm_view_plan* plan) {

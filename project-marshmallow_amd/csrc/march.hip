@@ -435,9 +435,9 @@ __device__ V3 atmosphere(const MarchConsts& c, V3 dir, V3 sunDir) {
 
 // IEEE division / square root in the intersection's arithmetic type.
 __device__ __forceinline__ float div_(float a, float b) { return __fdiv_rn(a, b); }
-__device__ __forceinline__ double div_(double a, double b) { return a / b; }
+__device__ __forceinline__ double div_(double a, double b) { return __ddiv_rn(a, b); }
 __device__ __forceinline__ float sqrtx(float a) { return __fsqrt_rn(a); }
-__device__ __forceinline__ double sqrtx(double a) { return sqrt(a); }
+__device__ __forceinline__ double sqrtx(double a) { return __dsqrt_rn(a); }
 
 // raySphereIntersection (comp:147-177) in arithmetic type I; a miss yields t = 0.
 template <class I>
@@ -857,9 +857,10 @@ __device__ __forceinline__ void run_batch(const MarchConsts& c, const MarchTextu
         }
         const unsigned long long km = __ballot(keep);
         if (km) {
-            unsigned base = 0;
-            if (lane == 0) base = atomicAdd(c.collectCount, (unsigned)__popcll(km));
-            base = __shfl(base, 0);
+            // every lane takes part in the atomic (lane 0 adds the batch's count, the others 0), so no
+            // divergent branch sits inside the march loop; lane 0's old value is the batch's base
+            const unsigned mine = atomicAdd(c.collectCount, lane == 0 ? (unsigned)__popcll(km) : 0u);
+            const unsigned base = __builtin_amdgcn_readlane(mine, 0);
             const unsigned slot = base + __builtin_amdgcn_mbcnt_hi((unsigned)(km >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
             if (keep && slot < c.collectCap) {
@@ -1133,53 +1134,44 @@ __global__ __launch_bounds__(256) void cloud_shadow_kernel(const mm::ShadowConst
     out[i] = accum;
 }
 
-// ------------------------------------------------------------------ multi-view persistent kernel
-// A fixed grid of waves (sized to fill every CU) drains one global queue of 64-pixel tiles that
-// spans all views: each wave claims the next tile with one atomic, finds its view by binary search
-// over the views' first-tile indices, and marches it with that view's constants and weather map.
-// Tiles are numbered view by view, so the tiles in flight at any time are a contiguous range
-// (mostly one view: its weather map stays cache-resident). Every wave leaves the loop once the
-// counter passes the last tile.
-__global__ void reset_counter_kernel(unsigned* ctr) {
-    if (threadIdx.x == 0) *ctr = 0u;
-}
-
+// ------------------------------------------------------------------ multi-view kernel
+// One launch over the groups of tiles of all views: block b takes group g = (b & 7) * per + b / 8
+// (each XCD gets a contiguous range of groups, so mostly one view's weather map per L2), finds its
+// view by binary search over the views' first-group indices, and marches the group with that
+// view's constants and weather map. The hardware dispatcher hands blocks to CUs as slots free up,
+// which is the global work queue; no counter or persistent loop is needed.
 template <int FSET, bool COUNT, int MODE>
-__device__ __forceinline__ void views_loop(const ViewDesc* __restrict__ views, int nViews, MarchTextures tx,
-                                           unsigned* __restrict__ ctr, int total,
-                                           unsigned long long* __restrict__ counts) {
-    for (;;) {
-        unsigned t = 0;
-        if (threadIdx.x == 0) t = atomicAdd(ctr, 1u);
-        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
-        if (t >= (unsigned)total) break;
-        int lo = 0, hi = nViews - 1;  // last view whose first tile is <= t
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (views[mid].tileBase <= (int)t) lo = mid;
-            else hi = mid - 1;
-        }
-        const ViewDesc& v = views[lo];
-        tx.weather = v.weather;
-        const int g = (int)t - v.tileBase;  // group of tilesPerWave tiles of this view
-        const int tile0 = g * v.c.tilesPerWave;
-        march_tiles<FSET, COUNT, MODE>(consts_ptr(&v.c), tx, tile0, min(v.c.tilesPerWave, v.c.tilesX * v.c.tilesY - tile0), v.out,
-                                       counts);
+__device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, int nViews, const MarchTextures& tx0,
+                                            int total, unsigned long long* __restrict__ counts) {
+    const int per = (total + 7) / 8;
+    const int g = xcd_tile(blockIdx.x, per);
+    if (g >= total) return;  // uniform across the block
+    int lo = 0, hi = nViews - 1;  // last view whose first group is <= g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (views[mid].tileBase <= g) lo = mid;
+        else hi = mid - 1;
     }
+    const ViewDesc& v = views[lo];
+    MarchTextures tx = tx0;
+    tx.weather = v.weather;
+    const int tile0 = (g - v.tileBase) * v.c.tilesPerWave;
+    march_tiles<FSET, COUNT, MODE>(consts_ptr(&v.c), tx, tile0, min(v.c.tilesPerWave, v.c.tilesX * v.c.tilesY - tile0),
+                                   v.out, counts);
 }
 
 template <int FSET, bool COUNT, int MODE>
-__global__ __launch_bounds__(64) void march_views_kernel(const ViewDesc* __restrict__ views, int nViews,
-                                                         const MarchTextures tx, unsigned* __restrict__ ctr, int total,
-                                                         unsigned long long* __restrict__ counts) {
-    views_loop<FSET, COUNT, MODE>(views, nViews, tx, ctr, total, counts);
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
+void march_views_kernel(const ViewDesc* __restrict__ views, int nViews, const MarchTextures tx, int total,
+                        unsigned long long* __restrict__ counts) {
+    views_group<FSET, COUNT, MODE>(views, nViews, tx, total, counts);
 }
 
 template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
-void march_views_surrogate_kernel(const ViewDesc* __restrict__ views, int nViews, const MarchTextures tx,
-                                  unsigned* __restrict__ ctr, int total, unsigned long long* __restrict__ counts) {
-    views_loop<FSET, COUNT, CONE_SURROGATE>(views, nViews, tx, ctr, total, counts);
+void march_views_surrogate_kernel(const ViewDesc* __restrict__ views, int nViews, const MarchTextures tx, int total,
+                                  unsigned long long* __restrict__ counts) {
+    views_group<FSET, COUNT, CONE_SURROGATE>(views, nViews, tx, total, counts);
 }
 
 template <int FSET, bool COUNT>
@@ -1231,29 +1223,16 @@ int launch_shadow(int fset, const ShadowConsts& c, const TexView& weather, const
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 
-int views_resident_waves(int fset, int mode, int* out) {
-    const void* k = fset == FSET_REF ? views_kernel_ptr_c<FSET_REF>(mode, false)
-                    : fset == FSET_FAST ? views_kernel_ptr_c<FSET_FAST>(mode, false)
-                                        : views_kernel_ptr_c<FSET_PREC>(mode, false);
-    if (!k) return MM_E_FORMAT;
-    int dev = 0, cus = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return MM_E_DEVICE;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return MM_E_DEVICE;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 64, 0) != hipSuccess) return MM_E_DEVICE;
-    *out = cus * (per > 0 ? per : 1);
-    return MM_OK;
-}
-
-int launch_views(int fset, int mode, const ViewDesc* views, int nViews, const MarchTextures& t, unsigned* ctr,
-                 int total, int grid, unsigned long long* counts, void* stream) {
+int launch_views(int fset, int mode, const ViewDesc* views, int nViews, const MarchTextures& t, int total,
+                 unsigned long long* counts, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     const void* k = fset == FSET_REF ? views_kernel_ptr_c<FSET_REF>(mode, counts != nullptr)
                     : fset == FSET_FAST ? views_kernel_ptr_c<FSET_FAST>(mode, counts != nullptr)
                                         : views_kernel_ptr_c<FSET_PREC>(mode, counts != nullptr);
     if (!k) return MM_E_FORMAT;
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, ctr);
-    void* args[] = {(void*)&views, (void*)&nViews, (void*)&t, (void*)&ctr, (void*)&total, (void*)&counts};
-    if (hipLaunchKernel(k, dim3((unsigned)grid), dim3(64), args, 0, s) != hipSuccess) return MM_E_DEVICE;
+    const unsigned grid = (unsigned)(8 * ((total + 7) / 8));
+    void* args[] = {(void*)&views, (void*)&nViews, (void*)&t, (void*)&total, (void*)&counts};
+    if (hipLaunchKernel(k, dim3(grid), dim3(64), args, 0, s) != hipSuccess) return MM_E_DEVICE;
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 

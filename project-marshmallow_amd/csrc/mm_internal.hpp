@@ -99,9 +99,8 @@ struct ViewDesc {
     int tileBase;                 // global index of this view's first group of tilesPerWave tiles
 };
 
-int views_resident_waves(int fset, int mode, int* out);
-int launch_views(int fset, int mode, const ViewDesc* views, int nViews, const MarchTextures& t, unsigned* ctr,
-                 int total, int grid, unsigned long long* counts, void* stream);
+int launch_views(int fset, int mode, const ViewDesc* views, int nViews, const MarchTextures& t, int total,
+                 unsigned long long* counts, void* stream);
 
 size_t surrogate_model_bytes();
 int surrogate_init(void* model, uint32_t seed, void* stream);

@@ -28,22 +28,28 @@ __device__ __forceinline__ F3 nrm(F3 a) {
     return f3(a.x * r, a.y * r, a.z * r);
 }
 
+// Correctly rounded division and square root in either precision (the oracle's `/` and `sqrt`).
+__device__ __forceinline__ float dv(float a, float b) { return __fdiv_rn(a, b); }
+__device__ __forceinline__ double dv(double a, double b) { return __ddiv_rn(a, b); }
+__device__ __forceinline__ float sq(float a) { return __fsqrt_rn(a); }
+__device__ __forceinline__ double sq(double a) { return __dsqrt_rn(a); }
+
 // `.point` of raySphereIntersection (reproject.comp:56-87), zero on a miss.
 template <class I>
 __device__ __forceinline__ void sphere_point(I rox, I roy, I roz, I rdx, I rdy, I rdz, I cx, I cy, I cz, I w, I& px,
                                              I& py, I& pz) {
-    rox = (rox - cx) / w;
-    roy = (roy - cy) / w;
-    roz = (roz - cz) / w;
+    rox = dv(rox - cx, w);
+    roy = dv(roy - cy, w);
+    roz = dv(roz - cz, w);
     const I A = (rdx * rdx + rdy * rdy) + rdz * rdz;
     const I B = I(2) * ((rdx * rox + rdy * roy) + rdz * roz);
     const I C = ((rox * rox + roy * roy) + roz * roz) - I(0.25);
     const I disc = (B * B) - ((I(4) * A) * C);
     px = py = pz = I(0);
     if (disc < I(0)) return;
-    const I sq = sqrt(disc);
-    I t = ((-sq - B) / A) * I(0.5);
-    if (t < I(0)) t = ((sq - B) / A) * I(0.5);
+    const I r = sq(disc);
+    I t = dv(-r - B, A) * I(0.5);
+    if (t < I(0)) t = dv(r - B, A) * I(0.5);
     if (!(t >= I(0))) return;
     px = (rox + rdx * t) * w + cx;
     py = (roy + rdy * t) * w + cy;

@@ -1,4 +1,4 @@
-"""Many views in one persistent launch (no counterpart in the reference): seeded view sets, HIP-graph
+"""Many views in one launch (no counterpart in the reference): seeded view sets, HIP-graph
 capture of the launch, and the autotuner that picks the tile shape and the coarse step length.
 
 View set (seeded per view): resolution log-uniform from 256x256 to 1920x1080, camera altitude
@@ -87,7 +87,7 @@ class ViewSet:
 
 
 def capture(plan, counts=None):
-    """HIP graph of one mm_march_views launch (counter reset + persistent kernel)."""
+    """HIP graph of one mm_march_views launch."""
     import torch
 
     g = torch.cuda.CUDAGraph()

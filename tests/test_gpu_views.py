@@ -1,4 +1,4 @@
-"""Many views in one persistent launch: each view's image equals its own single-view march, the
+"""Many views in one launch: each view's image equals its own single-view march, the
 launch replays correctly from a HIP graph, and the tile shape does not change the image."""
 import numpy as np
 import pytest
