@@ -952,9 +952,12 @@ __device__ __forceinline__ void march_tiles(ConstsPtr c0, const MarchTextures& t
         // constants would all stay live in scalar registers at once and spill into vector ones.
         ConstsPtr cp = c0;
         {
+            // readfirstlane returns a signed int: both halves go through uint32_t before widening,
+            // or a low half >= 2^31 would sign-extend over the high half
             const uint64_t v = (uint64_t)cp;
-            cp = (ConstsPtr)((((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32))) << 32) |
-                             __builtin_amdgcn_readfirstlane((uint32_t)v));
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+            cp = (ConstsPtr)(((uint64_t)hi << 32) | (uint64_t)lo);
         }
         asm volatile("" : "+s"(cp));
         const MarchConsts& c = *(const MarchConsts*)cp;

@@ -106,9 +106,10 @@ __device__ __forceinline__ FwdOut mlp_forward_tile(const uint4* __restrict__ fra
 template <class T>
 __device__ __forceinline__ T* uniform_ptr(T* p) {
     const uint64_t v = (uint64_t)p;
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return (T*)(((uint64_t)hi << 32) | lo);
+    // readfirstlane returns a signed int: keep both halves unsigned before widening
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T*)(((uint64_t)hi << 32) | (uint64_t)lo);
 }
 
 __device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
