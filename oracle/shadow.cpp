@@ -48,9 +48,12 @@ float layerDensity(float h, float type) {
     return mixf(d1, d2, type);
 }
 
-// model.frag:101-103
+// model.frag:101-103. pow(x, y) is evaluated as exp2(y * log2(x)) in double, rounded once to float
+// (GLSL defines pow through exp2/log2 and leaves its precision to the implementation; the HIP kernel
+// uses the same formula).
 float heightBiasCoverage(float coverage, float height) {
-    return powf(coverage, clampf(remap(height, 0.7f, 0.8f, 1.0f, 0.6f), 0.6f, 1.0f));
+    const float e = clampf(remap(height, 0.7f, 0.8f, 1.0f, 0.6f), 0.6f, 1.0f);
+    return (float)exp2((double)e * log2((double)coverage));
 }
 
 struct Ctx {

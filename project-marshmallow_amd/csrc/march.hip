@@ -16,6 +16,7 @@
 #include <hip/hip_fp16.h>
 #include "mm_internal.hpp"
 #include "surrogate_dev.hpp"
+#include "cr_math.hpp"
 
 namespace {
 
@@ -39,12 +40,12 @@ __device__ __forceinline__ V3 operator*(float s, V3 a) { return mk(s * a.x, s * 
 __device__ __forceinline__ V3 mul3(V3 a, V3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
 __device__ __forceinline__ float dot3(V3 a, V3 b) { return ((a.x * b.x) + (a.y * b.y)) + (a.z * b.z); }
 __device__ __forceinline__ V3 ld3(const float* f) { return mk(f[0], f[1], f[2]); }
-__device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float sqrt_(float x) { return sqrt_rn(x); }
 // normalize with a correctly rounded 1 / sqrt: getProjectedShellPoint scales the unit vector by the
 // shell radius (6.4e6 m in the Earth-radius scenario), so one ulp of the reciprocal moves the
 // projected point by ~0.5 m = 2e-4 of the shell height and flips density thresholds downstream.
-__device__ __forceinline__ V3 normalize3(V3 a) { return a * __fdiv_rn(1.0f, __fsqrt_rn(dot3(a, a))); }
-__device__ __forceinline__ float length3(V3 a) { return __fsqrt_rn(dot3(a, a)); }
+__device__ __forceinline__ V3 normalize3(V3 a) { return a * __fdiv_rn(1.0f, sqrt_rn(dot3(a, a))); }
+__device__ __forceinline__ float length3(V3 a) { return sqrt_rn(dot3(a, a)); }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 #ifndef MM_LERP_FMA
@@ -436,7 +437,7 @@ __device__ V3 atmosphere(const MarchConsts& c, V3 dir, V3 sunDir) {
 // IEEE division / square root in the intersection's arithmetic type.
 __device__ __forceinline__ float div_(float a, float b) { return __fdiv_rn(a, b); }
 __device__ __forceinline__ double div_(double a, double b) { return __ddiv_rn(a, b); }
-__device__ __forceinline__ float sqrtx(float a) { return __fsqrt_rn(a); }
+__device__ __forceinline__ float sqrtx(float a) { return sqrt_rn(a); }
 __device__ __forceinline__ double sqrtx(double a) { return __dsqrt_rn(a); }
 
 // raySphereIntersection (comp:147-177) in arithmetic type I; a miss yields t = 0.
@@ -1097,7 +1098,10 @@ __device__ float sh_cloud_test(const mm::ShadowConsts& c, const TexView& wt, con
     float density = layer * sh_remap_clamped(low_shape<FSET>(lg, lt), 0.3f, 1.0f, 0.0f, 1.0f);
     if (density < 0.0001f) return 0.0f;
     const float e = clampf(sh_remap(fminf(0.85f, w.coverage), 0.7f, 0.8f, 1.0f, 0.6f), 0.6f, 1.0f);
-    const float coverage = powg(h, e);
+    // pow evaluated as exp2(e * log2(h)) in double and rounded once (the scalar restatement uses the
+    // same formula): the shadow is not on the frame's critical path, and the single rounding keeps
+    // the 1e-4 early out and the erosion remap from amplifying last-bit differences
+    const float coverage = (float)exp2((double)e * log2((double)h));
     const float erosion = sh_remap_clamped(low_erosion<FSET>(lg, lt), coverage, 1.0f, 0.0f, 1.0f);
     return sh_remap_clamped(density, erosion, 1.0f, 0.0f, 1.0f);
 }

@@ -13,6 +13,7 @@
 #include <hip/hip_fp16.h>
 #include <float.h>
 #include "mm_internal.hpp"
+#include "cr_math.hpp"
 
 namespace {
 
@@ -239,7 +240,7 @@ __device__ float worley(float px, float py, float pz, int cells, uint32_t seed) 
             }
         }
     }
-    float d = sqrtf(best);
+    float d = sqrt_rn(best);
     return 1.0f - (d < 1.0f ? d : 1.0f);
 }
 

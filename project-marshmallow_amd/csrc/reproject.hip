@@ -12,6 +12,7 @@
 #include <math.h>
 
 #include "mm_internal.hpp"
+#include "cr_math.hpp"
 
 namespace {
 
@@ -24,14 +25,14 @@ __device__ __forceinline__ F3 sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, 
 __device__ __forceinline__ F3 scl(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
 __device__ __forceinline__ float dot(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 __device__ __forceinline__ F3 nrm(F3 a) {
-    const float r = __fdiv_rn(1.0f, __fsqrt_rn(dot(a, a)));
+    const float r = __fdiv_rn(1.0f, sqrt_rn(dot(a, a)));
     return f3(a.x * r, a.y * r, a.z * r);
 }
 
 // Correctly rounded division and square root in either precision (the oracle's `/` and `sqrt`).
 __device__ __forceinline__ float dv(float a, float b) { return __fdiv_rn(a, b); }
 __device__ __forceinline__ double dv(double a, double b) { return __ddiv_rn(a, b); }
-__device__ __forceinline__ float sq(float a) { return __fsqrt_rn(a); }
+__device__ __forceinline__ float sq(float a) { return sqrt_rn(a); }
 __device__ __forceinline__ double sq(double a) { return __dsqrt_rn(a); }
 
 // `.point` of raySphereIntersection (reproject.comp:56-87), zero on a miss.
