@@ -87,6 +87,51 @@ def cpu_baseline(mm, ns, cam, sun, sky, prm, W, H, budget_s):
             "ms_per_frame_projected": None if not samples else None}
 
 
+def surrogate_leg(mm, ns, cam, sun, sky, prm, W, H, steps, warmup, stream, train_steps):
+    """Learned light cone at this frame: a surrogate trained on other scenes (16 seeded views at
+    960x540, weather seeds 1001-1016, sun U[2, 70] deg, coverage U[0.2, 0.9]; outside the timed
+    region), then the surrogate-lit march timed like the plain one. The threshold is the largest of
+    a fixed ladder whose image stays within 1/255 mean per-channel error of the plain march."""
+    import torch
+    from marshmallow_amd import training
+
+    cfg = training.TrainConfig(width=960, height=540, train_seeds=tuple(range(1001, 1017)), rows_per_view=1 << 18,
+                               steps=train_steps, batch=8192, lr=2e-3, lr_final=2e-4)
+    model, info = training.train_surrogate(mm, cfg)
+    plain = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
+    surr = torch.empty_like(plain)
+    mm.march_clouds(cam, sun, sky, ns, plain, W, H, params=prm, stream=stream)
+    best = None
+    for thr in (0.0005, 0.001, 0.002, 0.005, 0.01, 0.02, 1e30):
+        ps = mm.march_params("scenario", cone_mode=mm.CONE_SURROGATE, surrogate_threshold=thr, surrogate=model.ptr)
+        c = torch.zeros(4, dtype=torch.int64, device="cuda")
+        mm.march_clouds(cam, sun, sky, ns, surr, W, H, params=ps, counts=c, stream=stream)
+        torch.cuda.synchronize()
+        mean, mx, frac = training.image_error_255(surr, plain)
+        c = [int(v) for v in c.cpu()]
+        if mean <= 1.0 or best is None:
+            best = {"threshold": thr, "mean_err_255": round(mean, 4), "max_err_255": round(mx, 3),
+                    "frac_within_1": round(frac, 5), "replaced_frac": round(1.0 - c[3] / max(c[2], 1), 4),
+                    "params": ps}
+    def timed(p):
+        for _ in range(warmup):
+            mm.march_clouds(cam, sun, sky, ns, surr, W, H, params=p, stream=stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            mm.march_clouds(cam, sun, sky, ns, surr, W, H, params=p, stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+    t_plain = timed(prm)
+    t_surr = timed(best["params"])
+    best.pop("params")
+    best.update({"within_1_255": best["mean_err_255"] <= 1.0, "march_ms_plain": round(t_plain, 4), "march_ms_surrogate": round(t_surr, 4),
+                 "speedup": round(t_plain / t_surr, 3), "training": info})
+    return best
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,8 +142,9 @@ def main():
     ap.add_argument("--fset", default="fast", choices=["fast", "ref", "prec"])
     ap.add_argument("--no-batch", action="store_true", help="evaluate light cones in place (lock-step kernel)")
     ap.add_argument("--tile-width", type=int, default=0, help="8, 16 or 32 (0 = library default)")
-    ap.add_argument("--tiles-per-wave", type=int, default=0, help="tiles per wave with ray refill (0 = default)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work (0 = skip)")
+    ap.add_argument("--surrogate-steps", type=int, default=4000,
+                    help="Adam steps of the learned light-cone surrogate leg (0 = skip the leg)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -129,7 +175,6 @@ def main():
     if args.no_batch:
         prm.flags |= mm.MARCH_NO_BATCH
     prm.tile_width = args.tile_width
-    prm.tiles_per_wave = args.tiles_per_wave
     ns = mm.NoiseSet(args.fset)
     nparams = mm.noise_params(seed=1, weather_seed=1, coverage=0.5)
     out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
@@ -191,7 +236,7 @@ def main():
                                f"FOV, sun 25 deg, coverage 0.5, t = 0, weather seed 1)",
                    "width": W, "height": H, "format_set": args.fset,
                    "kernel": "lock-step" if args.no_batch else "cone-batched", "coarse_steps_max": 128, "fine_steps_max": 256,
-                   "tile_width": args.tile_width or 8, "tiles_per_wave": args.tiles_per_wave or 1},
+                   "tile_width": args.tile_width or 8},
         "ms_per_frame": round(ms_step, 4),
         "frames_per_s": round(1000.0 / ms_step, 2),
         "march_ms": round(march_ms, 4),
@@ -202,7 +247,11 @@ def main():
         "wall_s": round(wall, 3),
         "roofline": roofline,
         "cpu_baseline": None,
+        "surrogate": None,
     }
+    if args.surrogate_steps > 0 and not args.no_batch:
+        rec["surrogate"] = surrogate_leg(mm, ns, cam, sun, sky, prm, W, H, args.steps, args.warmup, stream,
+                                         args.surrogate_steps)
     if args.cpu_budget > 0:
         cb = cpu_baseline(mm, ns, cam, sun, sky, prm, W, H, args.cpu_budget)
         cb["ms_per_frame_projected"] = round(n_density / cb["value"] * 1e3, 1)

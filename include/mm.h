@@ -192,9 +192,7 @@ typedef struct mm_march_params {
                                    REF and FAST texture sets only */
     int32_t tile_width;         /* pixels per wave tile row: 8 (8x8, default when 0), 16 (16x4), 32 (32x2);
                                    the image does not depend on it */
-    int32_t tiles_per_wave;     /* tiles one wave marches, refilling lanes whose ray has ended with the
-                                   next pixel (0 = default); the image does not depend on it */
-    int32_t _reserved[6];
+    int32_t _reserved[7];
 } mm_march_params;
 
 /* Preset with the reference's constants (compute-clouds.comp, bit-for-bit). */
@@ -284,8 +282,7 @@ int mm_cloud_shadow(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm
 /* ---------------------------------------------------------------- many views in one launch
  * No counterpart in the reference (which dispatches one view per frame, VulkanApplication.cpp:
  * 1061-1071): many views of any resolution marched by one launch over the 64-pixel tiles of all
- * views (grouped params.tiles_per_wave at a time), handed to the CUs by the hardware dispatcher as
- * slots free up. Views share the curl and noise volumes and bring their own uniforms, weather map
+ * views, handed to the CUs by the hardware dispatcher as slots free up. Views share the curl and noise volumes and bring their own uniforms, weather map
  * and output image. Each view's image is identical to mm_march_clouds of that view with the same
  * params. A plan holds the per-view constants in device memory; mm_march_views enqueues one kernel
  * (no allocation, no host transfer, no memset), so it can be captured into a HIP graph and
@@ -309,7 +306,7 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
 /* Replace the views' uniforms / outputs (same count) without reallocating. Synchronises the stream. */
 int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_views, const mm_march_params* params,
                         void* stream);
-/* Work items (groups of tiles_per_wave tiles) and the launch's grid (waves) of the plan. */
+/* Total 64-pixel tiles and the launch's grid (waves) of the plan. */
 int mm_view_plan_info(const mm_view_plan* plan, int32_t* total_tiles, int32_t* grid_waves);
 /* counts: NULL or device uint64[4] accumulators as for mm_march_clouds_counted. */
 int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* stream);

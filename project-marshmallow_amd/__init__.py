@@ -90,8 +90,7 @@ class MarchParams(ctypes.Structure):
                 ("wind_strength", ctypes.c_float), ("weather_scale", ctypes.c_float), ("low_scale", ctypes.c_float),
                 ("curl_scale", ctypes.c_float), ("hi_scale", ctypes.c_float), ("subtexel_bits", ctypes.c_int32),
                 ("cone_mode", ctypes.c_int32), ("surrogate_threshold", ctypes.c_float),
-                ("surrogate", ctypes.c_void_p), ("tile_width", ctypes.c_int32), ("tiles_per_wave", ctypes.c_int32),
-                ("_reserved", ctypes.c_int32 * 6)]
+                ("surrogate", ctypes.c_void_p), ("tile_width", ctypes.c_int32), ("_reserved", ctypes.c_int32 * 7)]
 
 
 class Scene(ctypes.Structure):

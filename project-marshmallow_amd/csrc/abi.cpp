@@ -90,9 +90,6 @@ mm::TexView view_of(const mm_texture& t) {
     return v;
 }
 
-// Tiles per wave of the batched march when params.tiles_per_wave is 0.
-constexpr int kDefaultTilesPerWave = 1;
-
 // hipGetLastError() reports the last error of any earlier runtime call on this thread, including
 // benign ones made by the caller's own runtime set-up; launch status is read with it, so it is
 // cleared before this library's launches.
@@ -110,7 +107,6 @@ int check_params(const mm_march_params* p) {
     if (p->subtexel_bits != 0 && p->subtexel_bits != 8) return MM_E_PARAM;
     if (p->cone_mode != MM_CONE_PLAIN && p->cone_mode != MM_CONE_SURROGATE) return MM_E_PARAM;
     if (p->tile_width != 0 && p->tile_width != 8 && p->tile_width != 16 && p->tile_width != 32) return MM_E_PARAM;
-    if (p->tiles_per_wave < 0 || p->tiles_per_wave > 64) return MM_E_PARAM;
     if (p->cone_mode == MM_CONE_SURROGATE && (!p->surrogate || !(p->surrogate_threshold >= 0.f)))
         return MM_E_PARAM;
     return MM_OK;
@@ -403,8 +399,6 @@ static int build_march(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const
     c.tilesX = (c.gridW + (1 << c.tileShift) - 1) >> c.tileShift;
     c.tilesY = (c.gridH + (64 >> c.tileShift) - 1) / (64 >> c.tileShift);
     c.tilesPerXcd = (c.tilesX * c.tilesY + 7) / 8;
-    c.tilesPerWave = params->tiles_per_wave > 0 ? params->tiles_per_wave : kDefaultTilesPerWave;
-    c.groupsPerXcd = ((c.tilesX * c.tilesY + c.tilesPerWave - 1) / c.tilesPerWave + 7) / 8;
 
     memset(&t, 0, sizeof(t));
     t.weather = view_of(tex->weather);
@@ -635,7 +629,7 @@ static int fill_views(const mm_view* views, int32_t n, const mm_noise_set* share
         d.weather = t.weather;
         d.out = v.out_rgba32f;
         d.tileBase = total;
-        const long long tiles = empty ? 0 : ((long long)c.tilesX * c.tilesY + c.tilesPerWave - 1) / c.tilesPerWave;
+        const long long tiles = empty ? 0 : (long long)c.tilesX * c.tilesY;
         if ((long long)total + tiles > (1ll << 30)) return MM_E_DIM;
         total += (int)tiles;
     }

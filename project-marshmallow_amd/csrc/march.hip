@@ -914,89 +914,33 @@ __device__ __forceinline__ void queue_shift(ConeQueue& q, const int nb, const in
     __syncthreads();
 }
 
-// Constant-address-space pointer: reads through it are scalar loads (uniform, scalar-cached).
-typedef const __attribute__((address_space(4))) MarchConsts* ConstsPtr;
-
-template <class T>
-__device__ __forceinline__ ConstsPtr consts_ptr(const T* p) {
-    return (ConstsPtr)(p);
-}
-
-constexpr int kRefillMin = 16;  // free lanes that interrupt the primary march to take new pixels
-
-// One wave marches the 64 * ntiles pixels of tiles [tile0, tile0 + ntiles) (of the marched grid,
-// (1 << tileShift) x (64 >> tileShift) pixels each, lane-major in x). Lanes whose ray has finished
-// take the next pixel of the range (ballot + mbcnt), so the primary march keeps the wave busy
-// across ray lengths; a pixel's output is written once its ray has ended and all its queued cone
-// jobs are folded in. Every pixel sees exactly the shader's operations in the shader's order, so
-// the image does not depend on ntiles or the tile shape.
+// One wave marches one 64-pixel tile ((1 << tileShift) x (64 >> tileShift) pixels of the marched
+// grid, lane-major in x). Lanes run their primary march in lock-step; light-cone jobs go to the LDS
+// queue and are evaluated 64 at a time (run_batch), each lane folding its results in creation order.
 template <int FSET, bool COUNT, int MODE>
-__device__ __forceinline__ void march_tiles(ConstsPtr c0, const MarchTextures& tx, const int tile0,
-                                            const int ntiles, float* __restrict__ out,
-                                            unsigned long long* __restrict__ counts) {
+__device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchTextures& tx, const int tile,
+                                           float* __restrict__ out, unsigned long long* __restrict__ counts) {
     __shared__ ConeQueue q;
     const int lane = threadIdx.x;
-    const int tw = 1 << ((const MarchConsts*)c0)->tileShift, th = 64 >> ((const MarchConsts*)c0)->tileShift;
-    const int total = ntiles * 64;  // pixel slots of this wave
-    int next = 0;                   // next pixel slot (wave-uniform)
+    const int gi = (tile % c.tilesX) * (1 << c.tileShift) + (lane & ((1 << c.tileShift) - 1));
+    const int gj = (tile / c.tilesX) * (64 >> c.tileShift) + (lane >> c.tileShift);
+    const bool valid = gi < c.gridW && gj < c.gridH;
+    int px = 0, py = 0;
+    if (valid) tile_pixel(c, gi, gj, px, py);
     int nd = 0, nh = 0, nj = 0, nc = 0;  // density, detail samples; cone jobs, jobs on the plain cone
     RaySetup r;
     MarchState m;
     m.active = false;
-    bool has = false;  // the lane holds a pixel whose output is not written yet
-    int px = 0, py = 0, pend = 0;
+    if (valid) {
+        r = ray_setup(c, tx, px, py);
+        if (!r.march)
+            reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = make_float4(r.bg.x, r.bg.y, r.bg.z, r.alpha);
+        march_init(c, r, m);
+    }
     float transmittance = 1.0f;
     int qn = 0;  // queued jobs (wave-uniform)
     for (;;) {
-        // The constants are re-read (scalar loads) every iteration through a pointer the optimiser
-        // cannot follow: hoisted out of the loop, the set-up constants of a refill and the march
-        // constants would all stay live in scalar registers at once and spill into vector ones.
-        ConstsPtr cp = c0;
-        {
-            // readfirstlane returns a signed int: both halves go through uint32_t before widening,
-            // or a low half >= 2^31 would sign-extend over the high half
-            const uint64_t v = (uint64_t)cp;
-            const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
-            const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-            cp = (ConstsPtr)(((uint64_t)hi << 32) | (uint64_t)lo);
-        }
-        asm volatile("" : "+s"(cp));
-        const MarchConsts& c = *(const MarchConsts*)cp;
-        // a pixel whose ray has ended and whose cone jobs are all folded in is complete
-        if (has && !m.active && pend == 0) {
-            reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
-            has = false;
-        }
-        // refill lanes without a pixel
-        const unsigned long long need = __ballot(!has);
-        if (need != 0ull && next < total) {
-            const int k = next + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(need >> 32),
-                                                                __builtin_amdgcn_mbcnt_lo((unsigned)need, 0u));
-            next += __popcll(need);
-            if (!has && k < total) {
-                const int tile = tile0 + (k >> 6), li = k & 63;
-                const int gi = (tile % c.tilesX) * tw + (li & (tw - 1)), gj = (tile / c.tilesX) * th + (li >> c.tileShift);
-                if (gi < c.gridW && gj < c.gridH) {
-                    tile_pixel(c, gi, gj, px, py);
-                    r = ray_setup(c, tx, px, py);
-                    if (!r.march) {
-                        reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] =
-                            make_float4(r.bg.x, r.bg.y, r.bg.z, r.alpha);
-                    } else {
-                        march_init(c, r, m);
-                        transmittance = 1.0f;
-                        pend = 0;
-                        has = true;
-                    }
-                }
-            }
-        }
-        if (!__any(has)) {  // then the queue is empty too (jobs belong to unwritten pixels)
-            if (next >= total) break;
-            continue;
-        }
-        // primary march until a full batch is queued, every ray has left its loop, or enough lanes
-        // are free to take new pixels
+        // primary march until a full batch is queued or every ray has left its loop
         while (qn < 64 && __any(m.active)) {
             V3 jp;
             float js = 0.f, jx0 = 0.f, jw = 0.f, jrh = 0.f, jlo = 0.f, jd = 0.f;
@@ -1009,28 +953,19 @@ __device__ __forceinline__ void march_tiles(ConstsPtr c0, const MarchTextures& t
                 q.x[slot] = jp.x; q.y[slot] = jp.y; q.z[slot] = jp.z; q.step[slot] = js;
                 q.x0[slot] = jx0; q.w[slot] = jw; q.blend[slot] = r.beerBlend; q.owner[slot] = lane;
                 q.rh[slot] = jrh; q.lo[slot] = jlo; q.dens[slot] = jd;
-                pend++;
             }
             qn += __popcll(mask);
-            if (next < total && __popcll(__ballot(!has || (!m.active && pend == 0))) >= kRefillMin) break;
         }
-        const bool canRefill = next < total && __any(!has || (!m.active && pend == 0));
-        // a batch runs when full; a partial one when nothing else can move, or when half the wave
-        // waits on queued cone jobs to complete its pixels
-        if (qn >= 64 || (qn > 0 && ((!__any(m.active) && !canRefill) ||
-                                    __popcll(__ballot(has && !m.active)) >= 32))) {
-            __syncthreads();
-            const int nb = qn < 64 ? qn : 64;
-            run_batch<FSET, COUNT, MODE>(c, tx, q, nb, nd, nh, nj, nc);
-            for (int s = 0; s < nb; s++)
-                if (q.owner[s] == lane) {
-                    transmittance = mixf(transmittance, q.res[s], q.w[s]);
-                    pend--;
-                }
-            queue_shift(q, nb, qn - nb);
-            qn -= nb;
-        }
+        if (qn == 0) break;
+        __syncthreads();
+        const int nb = qn < 64 ? qn : 64;
+        run_batch<FSET, COUNT, MODE>(c, tx, q, nb, nd, nh, nj, nc);
+        for (int s = 0; s < nb; s++)
+            if (q.owner[s] == lane) transmittance = mixf(transmittance, q.res[s], q.w[s]);
+        queue_shift(q, nb, qn - nb);
+        qn -= nb;
     }
+    if (valid && r.march) reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
     if (COUNT) {
         atomicAdd(&counts[0], (unsigned long long)nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
@@ -1042,14 +977,12 @@ __device__ __forceinline__ void march_tiles(ConstsPtr c0, const MarchTextures& t
 }
 
 template <int FSET, bool COUNT, int MODE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void march_batched_kernel(const MarchConsts c, const MarchTextures tx,
+__global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, const MarchTextures tx,
                                                            float* __restrict__ out,
                                                            unsigned long long* __restrict__ counts) {
-    const int g = xcd_tile(blockIdx.x, c.groupsPerXcd);
-    const int tile0 = g * c.tilesPerWave, ntiles = min(c.tilesPerWave, c.tilesX * c.tilesY - tile0);
-    if (ntiles <= 0) return;  // uniform across the block
-    // c is the first kernel argument: it sits at offset 0 of the kernel-argument segment
-    march_tiles<FSET, COUNT, MODE>(consts_ptr(__builtin_amdgcn_kernarg_segment_ptr()), tx, tile0, ntiles, out, counts);
+    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
+    if (tile >= c.tilesX * c.tilesY) return;  // uniform across the block
+    march_tile<FSET, COUNT, MODE>(c, tx, tile, out, counts);
 }
 
 #ifndef MM_SURROGATE_WAVES_PER_EU
@@ -1061,11 +994,9 @@ template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
 void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
                             unsigned long long* __restrict__ counts) {
-    const int g = xcd_tile(blockIdx.x, c.groupsPerXcd);
-    const int tile0 = g * c.tilesPerWave, ntiles = min(c.tilesPerWave, c.tilesX * c.tilesY - tile0);
-    if (ntiles <= 0) return;
-    march_tiles<FSET, COUNT, CONE_SURROGATE>(consts_ptr(__builtin_amdgcn_kernarg_segment_ptr()), tx, tile0, ntiles, out,
-                                             counts);
+    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
+    if (tile >= c.tilesX * c.tilesY) return;
+    march_tile<FSET, COUNT, CONE_SURROGATE>(c, tx, tile, out, counts);
 }
 
 // ------------------------------------------------------------------ cloud shadows (model.frag)
@@ -1142,10 +1073,10 @@ __global__ __launch_bounds__(256) void cloud_shadow_kernel(const mm::ShadowConst
 }
 
 // ------------------------------------------------------------------ multi-view kernel
-// One launch over the groups of tiles of all views: block b takes group g = (b & 7) * per + b / 8
-// (each XCD gets a contiguous range of groups, so mostly one view's weather map per L2), finds its
-// view by binary search over the views' first-group indices, and marches the group with that
-// view's constants and weather map. The hardware dispatcher hands blocks to CUs as slots free up,
+// One launch over the tiles of all views: block b takes tile g = (b & 7) * per + b / 8 (each XCD
+// gets a contiguous range of tiles, so mostly one view's weather map per L2), finds its view by
+// binary search over the views' first-tile indices, and marches the tile with that view's
+// constants and weather map. The hardware dispatcher hands blocks to CUs as slots free up,
 // which is the global work queue; no counter or persistent loop is needed.
 template <int FSET, bool COUNT, int MODE>
 __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, int nViews, const MarchTextures& tx0,
@@ -1153,7 +1084,7 @@ __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, 
     const int per = (total + 7) / 8;
     const int g = xcd_tile(blockIdx.x, per);
     if (g >= total) return;  // uniform across the block
-    int lo = 0, hi = nViews - 1;  // last view whose first group is <= g
+    int lo = 0, hi = nViews - 1;  // last view whose first tile is <= g
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (views[mid].tileBase <= g) lo = mid;
@@ -1162,13 +1093,11 @@ __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, 
     const ViewDesc& v = views[lo];
     MarchTextures tx = tx0;
     tx.weather = v.weather;
-    const int tile0 = (g - v.tileBase) * v.c.tilesPerWave;
-    march_tiles<FSET, COUNT, MODE>(consts_ptr(&v.c), tx, tile0, min(v.c.tilesPerWave, v.c.tilesX * v.c.tilesY - tile0),
-                                   v.out, counts);
+    march_tile<FSET, COUNT, MODE>(v.c, tx, g - v.tileBase, v.out, counts);
 }
 
 template <int FSET, bool COUNT, int MODE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ __launch_bounds__(64)
 void march_views_kernel(const ViewDesc* __restrict__ views, int nViews, const MarchTextures tx, int total,
                         unsigned long long* __restrict__ counts) {
     views_group<FSET, COUNT, MODE>(views, nViews, tx, total, counts);
@@ -1197,7 +1126,7 @@ const void* views_kernel_ptr_c(int mode, bool count) {
 
 template <int FSET, bool COUNT>
 int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts, hipStream_t s) {
-    dim3 grid((unsigned)(8 * ((c.flags & MM_MARCH_NO_BATCH) ? c.tilesPerXcd : c.groupsPerXcd))), block(64);
+    dim3 grid((unsigned)(8 * c.tilesPerXcd)), block(64);
     if (c.coneMode == CONE_PLAIN && (c.flags & MM_MARCH_NO_BATCH))
         hipLaunchKernelGGL((march_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
     else if (c.coneMode == CONE_PLAIN)

@@ -38,8 +38,6 @@ struct MarchConsts {
     int tileShift;                // tiles of (1 << tileShift) x (64 >> tileShift) pixels: 3, 4 or 5
     int tilesX, tilesY;           // tiles of the marched grid
     int tilesPerXcd;              // ceil(tilesX * tilesY / 8)
-    int tilesPerWave;             // tiles one wave of the batched kernel marches (with ray refill)
-    int groupsPerXcd;             // ceil(ceil(tilesX * tilesY / tilesPerWave) / 8)
     float probeOffsets[3][3];     // basis * (0, k, 0), k = 0.6, 2.5, 6: surrogate density probes
     // light-cone mode: MM_CONE_PLAIN, MM_CONE_SURROGATE, or 2 = plain + record surrogate training samples
     int coneMode;
@@ -96,7 +94,7 @@ struct ViewDesc {
     MarchConsts c;
     TexView weather;
     float* out;
-    int tileBase;                 // global index of this view's first group of tilesPerWave tiles
+    int tileBase;                 // global index of this view's first tile
 };
 
 int launch_views(int fset, int mode, const ViewDesc* views, int nViews, const MarchTextures& t, int total,

@@ -24,18 +24,17 @@ def _single(mm, vs, prm):
     return outs
 
 
-@pytest.mark.parametrize("tile_width,tpw", [(8, 0), (16, 0), (32, 0), (16, 4)])
-def test_views_equal_single_view_marches(mm, vs, tile_width, tpw):
+@pytest.mark.parametrize("tile_width", [8, 16, 32])
+def test_views_equal_single_view_marches(mm, vs, tile_width):
     import torch
 
-    prm = mm.march_params("scenario", tile_width=tile_width, tiles_per_wave=tpw)
+    prm = mm.march_params("scenario", tile_width=tile_width)
     ref = _single(mm, vs, mm.march_params("scenario"))
     for o in vs.outs:
         o.zero_()
     plan = mm.ViewPlan(vs.views(), vs.shared, prm)
     total, grid = plan.info()
-    k = max(tpw, 1)
-    assert total == sum(-(-(-(-s.width // tile_width) * -(-s.height // (64 // tile_width))) // k) for s in vs.specs)
+    assert total == sum(-(-s.width // tile_width) * -(-s.height // (64 // tile_width)) for s in vs.specs)
     assert grid > 0
     cnt = torch.zeros(4, dtype=torch.int64, device="cuda")
     plan.launch(cnt)
@@ -117,8 +116,8 @@ def test_tile_shape_does_not_change_the_image(mm, gpu):
     cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H))
     outs, cnts = [], []
     for tw in (8, 16, 32):
-        for flags, tpw in ((0, 0), (mm.MARCH_NO_BATCH, 0), (0, 3), (0, 8)):
-            prm = mm.march_params("scenario", tile_width=tw, tiles_per_wave=tpw)
+        for flags in (0, mm.MARCH_NO_BATCH):
+            prm = mm.march_params("scenario", tile_width=tw)
             prm.flags |= flags
             o = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
             c = torch.zeros(4, dtype=torch.int64, device="cuda")
