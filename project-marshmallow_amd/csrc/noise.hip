@@ -218,23 +218,54 @@ __device__ __forceinline__ uint32_t layer_seed(uint32_t seed, uint32_t layer) {
     return seed * 0x9E3779B9u + layer * 0x632BE5ABu;
 }
 
+// Sources of per-cell data for the Worley and Perlin bodies, addressed by unwrapped cell
+// coordinates. HashCells hashes the wrapped cell on the spot; LdsCells reads what a block staged
+// in LDS (the same hash of the same wrapped cell, computed once per block), so both feed the same
+// arithmetic and give the same bits.
+struct HashCells {
+    int cells;
+    uint32_t seed;
+    __device__ __forceinline__ void point(int x, int y, int z, float& ux, float& uy, float& uz) const {
+        uint32_t h0, h1, h2;
+        hash3((uint32_t)wrapi(x, cells), (uint32_t)wrapi(y, cells), (uint32_t)wrapi(z, cells), seed, h0, h1, h2);
+        ux = u01(h0);
+        uy = u01(h1);
+        uz = u01(h2);
+    }
+};
+
+struct LdsCells {
+    const float* a;
+    const float* b;
+    const float* c;
+    int bx, by, bz, nx, ny;  // first staged cell and the staged extent in x and y
+    __device__ __forceinline__ void point(int x, int y, int z, float& ux, float& uy, float& uz) const {
+        const int i = ((z - bz) * ny + (y - by)) * nx + (x - bx);
+        ux = a[i];
+        uy = b[i];
+        uz = c[i];
+    }
+};
+
 // Inverted F1 cellular noise on a periodic grid of `cells`^3 cells.
-__device__ float worley(float px, float py, float pz, int cells, uint32_t seed) {
+template <class Cells>
+__device__ __forceinline__ float worley_at(float px, float py, float pz, int cells, const Cells& src) {
     float qx = px * (float)cells, qy = py * (float)cells, qz = pz * (float)cells;
     int ix = (int)floorf(qx), iy = (int)floorf(qy), iz = (int)floorf(qz);
     float best = 1e30f;
+#pragma unroll
     for (int oz = -1; oz <= 1; oz++) {
-        const int wz = wrapi(iz + oz, cells);
         const float bz = qz - (float)(iz + oz);
+#pragma unroll
         for (int oy = -1; oy <= 1; oy++) {
-            const int wy = wrapi(iy + oy, cells);
             const float by = qy - (float)(iy + oy);
+#pragma unroll
             for (int ox = -1; ox <= 1; ox++) {
-                uint32_t h0, h1, h2;
-                hash3((uint32_t)wrapi(ix + ox, cells), (uint32_t)wy, (uint32_t)wz, seed, h0, h1, h2);
-                float dx = (qx - (float)(ix + ox)) - u01(h0);
-                float dy = by - u01(h1);
-                float dz = bz - u01(h2);
+                float ux, uy, uz;
+                src.point(ix + ox, iy + oy, iz + oz, ux, uy, uz);
+                float dx = (qx - (float)(ix + ox)) - ux;
+                float dy = by - uy;
+                float dz = bz - uz;
                 float d2 = ((dx * dx) + (dy * dy)) + (dz * dz);
                 best = d2 < best ? d2 : best;
             }
@@ -242,6 +273,10 @@ __device__ float worley(float px, float py, float pz, int cells, uint32_t seed) 
     }
     float d = sqrt_rn(best);
     return 1.0f - (d < 1.0f ? d : 1.0f);
+}
+
+__device__ float worley(float px, float py, float pz, int cells, uint32_t seed) {
+    return worley_at(px, py, pz, cells, HashCells{cells, seed});
 }
 
 __device__ float worley_fbm(float px, float py, float pz, int cells, uint32_t seed) {
@@ -254,28 +289,46 @@ __device__ float worley_fbm(float px, float py, float pz, int cells, uint32_t se
 __constant__ float kGrad[12][3] = {{1, 1, 0}, {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1},
                                    {1, 0, -1}, {-1, 0, -1}, {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1}};
 
+// Gradient of a lattice point of the periodic Perlin grid, from the hash of the wrapped point.
+struct HashGrads {
+    int period;
+    uint32_t seed;
+    __device__ __forceinline__ void point(int x, int y, int z, float& gx, float& gy, float& gz) const {
+        uint32_t h0, h1, h2;
+        hash3((uint32_t)wrapi(x, period), (uint32_t)wrapi(y, period), (uint32_t)wrapi(z, period), seed, h0, h1, h2);
+        const float* g = kGrad[h0 % 12u];
+        gx = g[0];
+        gy = g[1];
+        gz = g[2];
+    }
+};
+
 __device__ __forceinline__ float fade(float t) { return ((t * t) * t) * ((t * ((t * 6.0f) - 15.0f)) + 10.0f); }
 
-__device__ float perlin_tiled(float px, float py, float pz, int period, uint32_t seed) {
+template <class Grads>
+__device__ __forceinline__ float perlin_at(float px, float py, float pz, int period, const Grads& src) {
     float qx = px * (float)period, qy = py * (float)period, qz = pz * (float)period;
     float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
     float rx = qx - fx, ry = qy - fy, rz = qz - fz;
     int ix = (int)fx, iy = (int)fy, iz = (int)fz;
     float n[8];
+#pragma unroll
     for (int c = 0; c < 8; c++) {
         int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-        uint32_t h0, h1, h2;
-        hash3((uint32_t)wrapi(ix + dx, period), (uint32_t)wrapi(iy + dy, period), (uint32_t)wrapi(iz + dz, period), seed,
-              h0, h1, h2);
-        const float* g = kGrad[h0 % 12u];
+        float g0, g1, g2;
+        src.point(ix + dx, iy + dy, iz + dz, g0, g1, g2);
         float ax = rx - (float)dx, ay = ry - (float)dy, az = rz - (float)dz;
-        n[c] = ((g[0] * ax) + (g[1] * ay)) + (g[2] * az);
+        n[c] = ((g0 * ax) + (g1 * ay)) + (g2 * az);
     }
     float ux = fade(rx), uy = fade(ry), uz = fade(rz);
     float x00 = lerp_(n[0], n[1], ux), x10 = lerp_(n[2], n[3], ux);
     float x01 = lerp_(n[4], n[5], ux), x11 = lerp_(n[6], n[7], ux);
     float y0 = lerp_(x00, x10, uy), y1 = lerp_(x01, x11, uy);
     return lerp_(y0, y1, uz);
+}
+
+__device__ float perlin_tiled(float px, float py, float pz, int period, uint32_t seed) {
+    return perlin_at(px, py, pz, period, HashGrads{period, seed});
 }
 
 __device__ float perlin_fbm(float px, float py, float pz, int period, uint32_t seed) {
@@ -345,22 +398,74 @@ struct GenArgs {
     float inv_scale[4];
 };
 
-__global__ __launch_bounds__(256) void low_res_kernel(GenArgs a) {
-    const int sd = a.dim + a.apron;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)sd * sd * sd) return;
-    const int x = (int)(i % sd) % a.dim, y = (int)((i / sd) % sd) % a.dim, z = (int)(i / ((long)sd * sd)) % a.dim;
-    const float inv = 1.0f / (float)(2 * a.dim);
-    float px = (float)(2 * x + 1) * inv, py = (float)(2 * y + 1) * inv, pz = (float)(2 * z + 1) * inv;
-    float perlin = perlin_fbm(px, py, pz, 4, layer_seed(a.seed, 1));
+// ---- low-res volume: 4 Perlin octaves (periods 4..32) and 4 Worley layers of 3 octaves
+// (cells 4..128). "Level" l < 4 is Perlin octave l, l >= 4 is Worley layer (l - 4) / 3, octave
+// (l - 4) % 3; both sources below hand out the lattice data of a level.
+constexpr int kLowLevels = 16;
+__host__ __device__ __forceinline__ int low_level_cells(int l) {
+    return l < 4 ? 4 << l : (4 << ((l - 4) / 3)) << ((l - 4) % 3);
+}
+__device__ __forceinline__ uint32_t low_level_seed(uint32_t seed, int l) {
+    return l < 4 ? layer_seed(seed, 1) + (uint32_t)l : layer_seed(seed, 2 + (uint32_t)(l - 4) / 3) + (uint32_t)((l - 4) % 3);
+}
+
+struct LowResHash {
+    uint32_t seed;
+    __device__ __forceinline__ HashGrads perlin(int o) const { return HashGrads{low_level_cells(o), low_level_seed(seed, o)}; }
+    __device__ __forceinline__ HashCells worley(int L, int k) const {
+        const int l = 4 + 3 * L + k;
+        return HashCells{low_level_cells(l), low_level_seed(seed, l)};
+    }
+};
+
+// Lattice cells of one level staged for a brick: cells [b, b + n) per axis, at offset `off`.
+struct StageMeta {
+    int bx, by, bz, nx, ny, nz, off;
+};
+
+struct LowResLds {
+    const float* a;
+    const float* b;
+    const float* c;
+    const StageMeta* meta;
+    __device__ __forceinline__ LdsCells level(int l) const {
+        const StageMeta& m = meta[l];
+        return LdsCells{a + m.off, b + m.off, c + m.off, m.bx, m.by, m.bz, m.nx, m.ny};
+    }
+    __device__ __forceinline__ LdsCells perlin(int o) const { return level(o); }
+    __device__ __forceinline__ LdsCells worley(int L, int k) const { return level(4 + 3 * L + k); }
+};
+
+template <class Src>
+__device__ __forceinline__ float worley_fbm_at(float px, float py, float pz, int L, const Src& src) {
+    const int cells = 4 << L;
+    float a = worley_at(px, py, pz, cells, src.worley(L, 0));
+    float b = worley_at(px, py, pz, cells * 2, src.worley(L, 1));
+    float c = worley_at(px, py, pz, cells * 4, src.worley(L, 2));
+    return (0.625f * a + 0.25f * b) + 0.125f * c;
+}
+
+// The four channels of the low-res texel at (px, py, pz): Perlin-Worley, then Worley FBMs of
+// base frequency 8, 16 and 32.
+template <class Src>
+__device__ __forceinline__ void low_res_values(float px, float py, float pz, const Src& src, float v[4]) {
+    float s = 0.f, w = 1.f;
+#pragma unroll
+    for (int o = 0; o < 4; o++) {
+        s = s + w * perlin_at(px, py, pz, 4 << o, src.perlin(o));
+        w = w * 0.5f;
+    }
+    const float perlin = s * (1.0f / 1.875f);
     float p01 = clamp01(0.5f + 1.5f * perlin);
-    float w = worley_fbm(px, py, pz, 4, layer_seed(a.seed, 2));
-    float pw = (p01 - (w - 1.0f)) / (1.0f - (w - 1.0f));
-    float v[4];
+    float wf = worley_fbm_at(px, py, pz, 0, src);
+    float pw = (p01 - (wf - 1.0f)) / (1.0f - (wf - 1.0f));
     v[0] = clamp01(0.8f * pw - 0.04f);
-    v[1] = clamp01(0.26f + 0.9f * worley_fbm(px, py, pz, 8, layer_seed(a.seed, 3)));
-    v[2] = clamp01(0.26f + 0.9f * worley_fbm(px, py, pz, 16, layer_seed(a.seed, 4)));
-    v[3] = clamp01(0.26f + 0.9f * worley_fbm(px, py, pz, 32, layer_seed(a.seed, 5)));
+    v[1] = clamp01(0.26f + 0.9f * worley_fbm_at(px, py, pz, 1, src));
+    v[2] = clamp01(0.26f + 0.9f * worley_fbm_at(px, py, pz, 2, src));
+    v[3] = clamp01(0.26f + 0.9f * worley_fbm_at(px, py, pz, 3, src));
+}
+
+__device__ __forceinline__ void store_low_res(const GenArgs& a, long i, const float v[4]) {
     if (a.fmt == MM_FMT_RGBA8) {
         reinterpret_cast<uint32_t*>(a.dst)[i] = q8(v[0]) | (q8(v[1]) << 8) | (q8(v[2]) << 16) | (q8(v[3]) << 24);
     } else if (a.fmt == MM_FMT_R8E16) {
@@ -370,6 +475,97 @@ __global__ __launch_bounds__(256) void low_res_kernel(GenArgs a) {
         uint32_t r = 0;
         for (int k = 0; k < 4; k++) r |= fp8_e4m3(v[k] * a.inv_scale[k]) << (8 * k);
         reinterpret_cast<uint32_t*>(a.dst)[i] = r;
+    }
+}
+
+// Brick kernel: a block of 256 work-items takes an 8 x 8 x 4 brick of logical texels. The lattice
+// cells the brick's texels touch at each level (about 1.2k cells for a 128^3 volume, against 356
+// hashes per texel) are hashed once into LDS; the texels then read them there. When a brick
+// touches more cells than LDS holds (small volumes, where cells outnumber texels) it hashes in
+// place. Texels whose coordinate is below the apron width are also written to the apron copy
+// (x + dim, ...).
+constexpr int kBrickX = 8, kBrickY = 8, kBrickZ = 4;
+constexpr int kStageCap = 1536;  // staged cells per block: 18 KB of LDS
+
+// Cells [floor(q(t0)) - pad, floor(q(t1)) + 1] of one axis, q(t) = ((2t + 1) / (2 dim)) * cells,
+// evaluated as the texel evaluates it (q is monotone in t, so every texel's cells lie inside).
+__device__ __forceinline__ void axis_span(int t0, int t1, float inv, int cells, int pad, int& base, int& n) {
+    const float q0 = ((float)(2 * t0 + 1) * inv) * (float)cells;
+    const float q1 = ((float)(2 * t1 + 1) * inv) * (float)cells;
+    base = (int)floorf(q0) - pad;
+    n = ((int)floorf(q1) + 1) - base + 1;
+}
+
+__global__ __launch_bounds__(256) void low_res_brick_kernel(GenArgs a) {
+    __shared__ float sa[kStageCap], sb[kStageCap], sc[kStageCap];
+    __shared__ StageMeta meta[kLowLevels];
+    __shared__ int total;
+    const int tid = threadIdx.x;
+    const int nbx = (a.dim + kBrickX - 1) / kBrickX, nby = (a.dim + kBrickY - 1) / kBrickY;
+    const int x0 = (int)(blockIdx.x % nbx) * kBrickX, y0 = (int)((blockIdx.x / nbx) % nby) * kBrickY;
+    const int z0 = (int)(blockIdx.x / (nbx * nby)) * kBrickZ;
+    const int x1 = min(x0 + kBrickX, a.dim) - 1, y1 = min(y0 + kBrickY, a.dim) - 1, z1 = min(z0 + kBrickZ, a.dim) - 1;
+    const float inv = 1.0f / (float)(2 * a.dim);
+    if (tid < kLowLevels) {
+        const int cells = low_level_cells(tid), pad = tid < 4 ? 0 : 1;
+        StageMeta m;
+        axis_span(x0, x1, inv, cells, pad, m.bx, m.nx);
+        axis_span(y0, y1, inv, cells, pad, m.by, m.ny);
+        axis_span(z0, z1, inv, cells, pad, m.bz, m.nz);
+        m.off = m.nx * m.ny * m.nz;  // count for now; prefix-summed below
+        meta[tid] = m;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int off = 0;
+        for (int l = 0; l < kLowLevels; l++) {
+            const int cnt = meta[l].off;
+            meta[l].off = off;
+            off += cnt;
+        }
+        total = off;
+    }
+    __syncthreads();
+    const int nst = total;
+    const bool staged = nst <= kStageCap;  // uniform across the block
+    if (staged) {
+        for (int e = tid; e < nst; e += 256) {
+            int l = 0;
+            while (l + 1 < kLowLevels && meta[l + 1].off <= e) l++;
+            const StageMeta m = meta[l];
+            const int r = e - m.off, jx = r % m.nx, jy = (r / m.nx) % m.ny, jz = r / (m.nx * m.ny);
+            float u0, u1, u2;
+            if (l < 4)
+                HashGrads{low_level_cells(l), low_level_seed(a.seed, l)}.point(m.bx + jx, m.by + jy, m.bz + jz, u0, u1, u2);
+            else
+                HashCells{low_level_cells(l), low_level_seed(a.seed, l)}.point(m.bx + jx, m.by + jy, m.bz + jz, u0, u1, u2);
+            sa[e] = u0;
+            sb[e] = u1;
+            sc[e] = u2;
+        }
+    }
+    __syncthreads();
+    const int x = x0 + (tid & (kBrickX - 1)), y = y0 + ((tid / kBrickX) & (kBrickY - 1)), z = z0 + tid / (kBrickX * kBrickY);
+    if (x > x1 || y > y1 || z > z1) return;
+    const float px = (float)(2 * x + 1) * inv, py = (float)(2 * y + 1) * inv, pz = (float)(2 * z + 1) * inv;
+    float v[4];
+    if (staged)
+        low_res_values(px, py, pz, LowResLds{sa, sb, sc, meta}, v);
+    else
+        low_res_values(px, py, pz, LowResHash{a.seed}, v);
+    const long sd = a.dim + a.apron;
+    for (int cz = 0; cz < 2; cz++) {
+        const int zs = z + cz * a.dim;
+        if (cz && zs >= sd) break;
+        for (int cy = 0; cy < 2; cy++) {
+            const int ys = y + cy * a.dim;
+            if (cy && ys >= sd) break;
+            for (int cx = 0; cx < 2; cx++) {
+                const int xs = x + cx * a.dim;
+                if (cx && xs >= sd) break;
+                store_low_res(a, ((long)zs * sd + ys) * sd + xs, v);
+            }
+        }
     }
 }
 
@@ -488,8 +684,9 @@ int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* wor
         GenArgs a{(uint8_t*)t.data, t.width, t.layout == MM_LAYOUT_APRON ? 1 : 0, t.format, p->seed, 0.f, {1, 1, 1, 1}};
         if (t.format == MM_FMT_FP8X4)
             for (int k = 0; k < 4; k++) a.inv_scale[k] = 1.0f / t.scale[k];
-        const long sd = t.width + a.apron;
-        hipLaunchKernelGGL(low_res_kernel, dim3(blocks_for(sd * sd * sd)), dim3(256), 0, s, a);
+        const long bricks = (long)((t.width + kBrickX - 1) / kBrickX) * ((t.width + kBrickY - 1) / kBrickY) *
+                            ((t.width + kBrickZ - 1) / kBrickZ);
+        hipLaunchKernelGGL(low_res_brick_kernel, dim3((unsigned)bricks), dim3(256), 0, s, a);
     }
     if (p->what & MM_GEN_HI_RES) {
         const mm_texture& t = out->hi_res;

@@ -83,6 +83,20 @@ def test_generators_precision_formats(mm, gpu, ref):
     assert _bytes(ns.curl) == F.apron(h, 2).tobytes()
 
 
+@pytest.mark.parametrize("dim", [130, 36])
+def test_low_res_ragged_dims_bit_exact(mm, gpu, orc, dim):
+    # 130: partial bricks in every axis, apron copies, and bricks on both sides of the LDS staging
+    # limit in one launch; 36: more lattice cells than texels, every brick hashes in place
+    import torch
+
+    want = orc.low_res(dim, 11)
+    for fset, fmt, layout in (("ref", F.FMT_RGBA8, F.LAYOUT_LINEAR), ("fast", F.FMT_R8E16, F.LAYOUT_APRON)):
+        ns = mm.NoiseSet(fset, low_dim=dim)
+        ns.generate(mm.noise_params(seed=11, what=mm.GEN_LOW_RES), mm.current_stream())
+        torch.cuda.synchronize()
+        assert _bytes(ns.low_res) == F.pack_low(want, fmt, layout), (fset, dim)
+
+
 def test_fp8_encoder_matches_table(mm, gpu):
     # every e4m3 code decodes to a value that re-encodes to itself, ties go to the even code
     vals = F.fp8_e4m3_decode(np.arange(0x7F))
