@@ -132,6 +132,29 @@ def surrogate_leg(mm, ns, cam, sun, sky, prm, W, H, steps, warmup, stream, train
     return best
 
 
+def post_leg(mm, cam, sun, out, W, H, steps, warmup):
+    """Post chain (god rays -> radial blur -> tonemap, mm_post_process) on the marched frame, timed on
+    its own. Algorithmic HBM bytes per pixel: 16 read + 4 written (god rays), 16 + 4 read + 4 written
+    (radial blur + tonemap to RGBA8)."""
+    import torch
+
+    alpha = torch.empty(W * H, dtype=torch.float32, device="cuda")
+    rgba8 = torch.empty(W * H * 4, dtype=torch.uint8, device="cuda")
+    for _ in range(warmup):
+        mm.post_process(cam, sun, out, W, H, alpha, None, rgba8)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        mm.post_process(cam, sun, out, W, H, alpha, None, rgba8)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    alg = W * H * (16 + 4 + 16 + 4 + 4)
+    return {"post_ms": round(ms, 4), "algorithmic_bytes": alg, "achieved_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +166,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="evaluate light cones in place (lock-step kernel)")
     ap.add_argument("--tile-width", type=int, default=0, help="8, 16 or 32 (0 = library default)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work (0 = skip)")
+    ap.add_argument("--no-post", action="store_true", help="skip the post-chain leg")
     ap.add_argument("--surrogate-steps", type=int, default=4000,
                     help="Adam steps of the learned light-cone surrogate leg (0 = skip the leg)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -248,7 +272,10 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None,
         "surrogate": None,
+        "post": None,
     }
+    if not args.no_post:
+        rec["post"] = post_leg(mm, cam, sun, out, W, H, args.steps, args.warmup)
     if args.surrogate_steps > 0 and not args.no_batch:
         rec["surrogate"] = surrogate_leg(mm, ns, cam, sun, sky, prm, W, H, args.steps, args.warmup, stream,
                                          args.surrogate_steps)

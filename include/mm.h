@@ -38,7 +38,8 @@ enum {
     MM_E_PARAM = -6,     /* non-finite uniforms, zero-length sun direction, bad params */
     MM_E_DEVICE = -7,    /* HIP runtime error or no device */
     MM_E_SIZE = -8,      /* a texture allocation is smaller than its format requires */
-    MM_E_VERSION = -9    /* params.size does not match this library's struct */
+    MM_E_VERSION = -9,   /* params.size does not match this library's struct */
+    MM_E_IO = -10        /* a file could not be opened or read */
 };
 const char* mm_strerror(int code);
 int mm_abi_version(void);
@@ -278,6 +279,53 @@ void mm_shadow_params_reference(mm_shadow_params* p);
 int mm_cloud_shadow(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
                     const mm_texture* weather, const mm_texture* low_res, const float* points, int32_t n,
                     float* out, const mm_shadow_params* params, void* stream);
+
+/* ---------------------------------------------------------------- post chain
+ * Replaces the offscreen post passes that consume the cloud image (VulkanApplication.cpp:938-966:
+ * background copy -> god rays -> radial blur -> [meshes, not part of this path] -> tonemap; every
+ * framebuffer RGBA32F, sampled LINEAR / CLAMP_TO_EDGE, VulkanApplication.cpp:1290-1308):
+ *   god rays    (god-ray.frag:41-76)    alpha = 0.5 a + 8 decayed alpha taps toward the sun
+ *   radial blur (radialBlur.frag:36-63) rgb = sun.color * intensity * mean(10 alpha taps * 1.1) + 0.5 rgb
+ *   tonemap     (tonemap.frag:11-34)    Uncharted-2 curve, white point, gamma 1/2.2, vignette
+ * image: device RGBA32F cloud image (the march or reprojection output), row_pitch_px pixels per row
+ * (the same pitch for every image here). god_alpha: device float scratch, one per pixel (framebuffer
+ * 1's alpha). out_hdr_rgba32f (optional): the radial-blur output (framebuffer 2 before meshes).
+ * out_rgba8 (optional): the tonemapped frame as UNORM8 R, G, B, A = 255 (the swapchain is
+ * B8G8R8A8_UNORM, VulkanApplication.cpp:1439; the channel order here is RGBA). */
+typedef struct mm_post_params {
+    uint32_t size;              /* sizeof(mm_post_params) */
+    uint32_t flags;             /* none defined: 0 */
+    int32_t subtexel_bits;      /* sampler sub-texel precision, 1..8 (8 = the hardware's) */
+    int32_t god_ray_samples;    /* NUM_SAMPLES = 8 (god-ray.frag:31) */
+    float god_ray_density;      /* DENSITY = 0.75 */
+    float god_ray_decay;        /* DECAY = 0.99 */
+    float god_ray_exposure;     /* EXPOSURE = 0.9 */
+    float blur_scale;           /* 1.5 (radialBlur.frag:59) */
+    float blur_gain;            /* 1.1 */
+    float exposure;             /* 0.7 (tonemap.frag:30) */
+    float inv_gamma;            /* 1 / 2.2 */
+    float white_point;          /* 50.2 */
+    float vignette[3];          /* (0.1, 0.05, 0.13) */
+    int32_t _reserved[7];
+} mm_post_params;
+
+void mm_post_params_reference(mm_post_params* p);
+int mm_post_process(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const float* image_rgba32f,
+                    int32_t width, int32_t height, int32_t row_pitch_px, float* god_alpha,
+                    float* out_hdr_rgba32f, uint8_t* out_rgba8, const mm_post_params* params, void* stream);
+
+/* ---------------------------------------------------------------- asset ingestion
+ * Replaces the reference's texture loading: stb_image 2.16 with STBI_rgb_alpha for 2D images
+ * (Texture.cpp:212-246) and for volumes stored as one TGA per z slice, "<prefix>(<i>).tga"
+ * (Texture.cpp:502-538). Decodes on the host into caller memory as RGBA8, rows top first, slice i
+ * at byte offset i * width * height * 4; upload it into an RGBA8 LINEAR texture and convert it to
+ * another format set with mm_pack_texture. TGA: types 1/2/3/9/10/11, 8/15/16/24/32-bit pixels.
+ * PNG: all colour types, bit depths 1-16 (16-bit samples keep their high byte, as stb's 8-bit
+ * interface does), tRNS; interlaced PNGs are rejected (MM_E_FORMAT). */
+int mm_image_info(const char* path, int32_t* width, int32_t* height);
+int mm_load_image_rgba8(const char* path, uint8_t* rgba, size_t bytes, int32_t* width, int32_t* height);
+int mm_load_volume_rgba8(const char* prefix, int32_t depth, uint8_t* rgba, size_t bytes, int32_t* width,
+                         int32_t* height);
 
 /* ---------------------------------------------------------------- many views in one launch
  * No counterpart in the reference (which dispatches one view per frame, VulkanApplication.cpp:

@@ -10,6 +10,7 @@
  *                     Texture.cpp:29-52, 315-338 (LINEAR filter, REPEAT wrap, RGBA8 UNORM)
  *   - orc_reproject   reproject.comp:56-153 (temporal reprojection of the previous frame)
  *   - orc_cloud_shadow model.frag:52-129, 240-283 (the mesh pass's cloud-shadow march)
+ *   - orc_post_*      god-ray.frag:41-76, radialBlur.frag:36-63, tonemap.frag:11-34 (post chain)
  * Built with -ffp-contract=off and no fast-math so every float operation rounds as written.
  */
 #ifndef MM_ORACLE_H
@@ -95,6 +96,12 @@ float orc_shadow_layer_density(float h, float type);   /* model.frag:88-98 */
 int orc_cloud_shadow(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky, const uint8_t* weather,
                      int ww, int wh, const uint8_t* low, int ln, const float* points, int n, float* out,
                      const mm_shadow_params* p);
+
+/* ---- post chain (god-ray.frag, radialBlur.frag, tonemap.frag) ---- */
+void orc_post_sun_ndc(const mm_camera_ubo* cam, const mm_sun_ubo* sun, float out_xy[2]); /* (proj*view)*location / w */
+float orc_post_uc2(float x);                                                         /* tonemap.frag:11-14 */
+int orc_post_process(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const float* image, int width, int height,
+                     int pitch, float* god_alpha, float* hdr, uint8_t* rgba8, const mm_post_params* p);
 
 #ifdef __cplusplus
 }

@@ -73,6 +73,9 @@ class Oracle:
             "orc_reproject": (i, [vp, vp, vp, vp, i, i, i, vp, i, i]),
             "orc_cloud_shadow": (i, [vp, vp, vp, vp, i, i, vp, i, vp, i, vp, vp]),
             "orc_shadow_layer_density": (f, [f, f]),
+            "orc_post_sun_ndc": (None, [vp, vp, vp]),
+            "orc_post_uc2": (f, [f]),
+            "orc_post_process": (i, [vp, vp, vp, i, i, i, vp, vp, vp, vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -147,6 +150,25 @@ class Oracle:
         if r != 0:
             raise RuntimeError(f"oracle reproject failed: {r}")
         return out
+
+    def post_process(self, cam, sun, image: np.ndarray, params):
+        """The post chain restated on an (h, w, 4) float32 image: (god-ray alpha (h, w), radial-blur
+        RGBA (h, w, 4), tonemapped RGBA8 (h, w, 4))."""
+        img = np.ascontiguousarray(image, np.float32)
+        h, w = img.shape[:2]
+        alpha = np.zeros((h, w), np.float32)
+        hdr = np.zeros((h, w, 4), np.float32)
+        out = np.zeros((h, w, 4), np.uint8)
+        r = self.L.orc_post_process(ctypes.addressof(cam), ctypes.addressof(sun), _ptr(img), w, h, w, _ptr(alpha),
+                                    _ptr(hdr), _ptr(out), ctypes.addressof(params))
+        if r != 0:
+            raise RuntimeError(f"oracle post chain failed: {r}")
+        return alpha, hdr, out
+
+    def post_sun_ndc(self, cam, sun):
+        xy = np.zeros(2, np.float32)
+        self.L.orc_post_sun_ndc(ctypes.addressof(cam), ctypes.addressof(sun), _ptr(xy))
+        return xy
 
     def cloud_shadow(self, cam, sun, sky, weather: np.ndarray, low: np.ndarray, points: np.ndarray, params):
         """model.frag's shadow march restated: RGBA8 weather (h, w, 4) and low-res (n, n, n, 4)."""

@@ -23,9 +23,10 @@ LIB_PATH = os.environ.get("MM_LIBRARY", os.path.join(_HERE, "libmm.so"))
 MM_OK = 0
 ERRORS = {
     -1: "MM_E_NULL", -2: "MM_E_DIM", -3: "MM_E_PHASE", -4: "MM_E_PITCH", -5: "MM_E_FORMAT",
-    -6: "MM_E_PARAM", -7: "MM_E_DEVICE", -8: "MM_E_SIZE", -9: "MM_E_VERSION",
+    -6: "MM_E_PARAM", -7: "MM_E_DEVICE", -8: "MM_E_SIZE", -9: "MM_E_VERSION", -10: "MM_E_IO",
 }
-MM_E_NULL, MM_E_DIM, MM_E_PHASE, MM_E_PITCH, MM_E_FORMAT, MM_E_PARAM, MM_E_DEVICE, MM_E_SIZE, MM_E_VERSION = range(-1, -10, -1)
+(MM_E_NULL, MM_E_DIM, MM_E_PHASE, MM_E_PITCH, MM_E_FORMAT, MM_E_PARAM, MM_E_DEVICE, MM_E_SIZE, MM_E_VERSION,
+ MM_E_IO) = range(-1, -11, -1)
 
 FMT_RGBA8, FMT_RG8, FMT_R8E16, FMT_E16, FMT_FP8X4, FMT_FP16X4 = 1, 2, 3, 4, 5, 6
 LAYOUT_LINEAR, LAYOUT_APRON = 0, 1
@@ -109,6 +110,14 @@ class ShadowParams(ctypes.Structure):
                 ("subtexel_bits", ctypes.c_int32), ("_reserved", ctypes.c_int32 * 7)]
 
 
+class PostParams(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("subtexel_bits", ctypes.c_int32),
+                ("god_ray_samples", ctypes.c_int32), ("god_ray_density", ctypes.c_float),
+                ("god_ray_decay", ctypes.c_float), ("god_ray_exposure", ctypes.c_float), ("blur_scale", ctypes.c_float),
+                ("blur_gain", ctypes.c_float), ("exposure", ctypes.c_float), ("inv_gamma", ctypes.c_float),
+                ("white_point", ctypes.c_float), ("vignette", ctypes.c_float * 3), ("_reserved", ctypes.c_int32 * 7)]
+
+
 class View(ctypes.Structure):
     _fields_ = [("camera", CameraUBO), ("sun", SunUBO), ("sky", SkyUBO), ("weather", Texture),
                 ("out_rgba32f", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
@@ -142,6 +151,15 @@ _SIGS = {
     "mm_cloud_shadow": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(Texture), _P(Texture),
                                        ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, _P(ShadowParams),
                                        ctypes.c_void_p]),
+    "mm_post_params_reference": (None, [_P(PostParams)]),
+    "mm_post_process": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       _P(PostParams), ctypes.c_void_p]),
+    "mm_image_info": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_int32), _P(ctypes.c_int32)]),
+    "mm_load_image_rgba8": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_size_t, _P(ctypes.c_int32),
+                                           _P(ctypes.c_int32)]),
+    "mm_load_volume_rgba8": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t,
+                                            _P(ctypes.c_int32), _P(ctypes.c_int32)]),
     "mm_view_plan_create": (ctypes.c_int, [_P(View), ctypes.c_int32, _P(NoiseSetC), _P(MarchParams),
                                            _P(ctypes.c_void_p), ctypes.c_void_p]),
     "mm_view_plan_update": (ctypes.c_int, [ctypes.c_void_p, _P(View), ctypes.c_int32, _P(MarchParams),
@@ -375,6 +393,66 @@ def cloud_shadow(cam: CameraUBO, sun: SunUBO, sky: SkyUBO, noise: NoiseSet, poin
     check(lib().mm_cloud_shadow(ctypes.byref(cam), ctypes.byref(sun), ctypes.byref(sky), ctypes.byref(noise.weather.desc),
                                 ctypes.byref(noise.low_res.desc), points.data_ptr(), int(points.shape[0]),
                                 out.data_ptr(), pp, stream), "mm_cloud_shadow")
+
+
+def post_params(**overrides) -> PostParams:
+    p = PostParams()
+    lib().mm_post_params_reference(ctypes.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def post_process(cam: CameraUBO, sun: SunUBO, image, width: int, height: int, god_alpha, hdr=None, rgba8=None,
+                 pitch: int | None = None, params: PostParams | None = None, stream: int | None = None) -> None:
+    """God rays -> radial blur -> tonemap of a device RGBA32F cloud image (mm_post_process).
+    god_alpha: device float32 scratch [pitch * height]; hdr: optional float32 [pitch * height * 4];
+    rgba8: optional uint8 [pitch * height * 4]."""
+    stream = current_stream() if stream is None else stream
+    pitch = width if pitch is None else pitch
+    pp = ctypes.byref(params) if params is not None else None
+    check(lib().mm_post_process(ctypes.byref(cam), ctypes.byref(sun), image.data_ptr(), width, height, pitch,
+                                god_alpha.data_ptr(), hdr.data_ptr() if hdr is not None else None,
+                                rgba8.data_ptr() if rgba8 is not None else None, pp, stream), "mm_post_process")
+
+
+def load_image(path: str):
+    """Decode a TGA or PNG file to an (h, w, 4) uint8 array (mm_load_image_rgba8: stb_image's
+    STBI_rgb_alpha semantics, rows top first)."""
+    import numpy as np
+
+    w, h = ctypes.c_int32(), ctypes.c_int32()
+    check(lib().mm_image_info(path.encode(), ctypes.byref(w), ctypes.byref(h)), "mm_image_info")
+    a = np.zeros((h.value, w.value, 4), np.uint8)
+    check(lib().mm_load_image_rgba8(path.encode(), a.ctypes.data, a.nbytes, ctypes.byref(w), ctypes.byref(h)),
+          "mm_load_image_rgba8")
+    return a
+
+
+def load_volume(prefix: str, depth: int):
+    """Decode the TGA slices "<prefix>(<i>).tga", i < depth, to a (depth, h, w, 4) uint8 array
+    (mm_load_volume_rgba8, the reference's Texture3D::initFromFile layout)."""
+    import numpy as np
+
+    w, h = ctypes.c_int32(), ctypes.c_int32()
+    check(lib().mm_image_info(f"{prefix}(0).tga".encode(), ctypes.byref(w), ctypes.byref(h)), "mm_image_info")
+    a = np.zeros((depth, h.value, w.value, 4), np.uint8)
+    check(lib().mm_load_volume_rgba8(prefix.encode(), depth, a.ctypes.data, a.nbytes, ctypes.byref(w),
+                                     ctypes.byref(h)), "mm_load_volume_rgba8")
+    return a
+
+
+def upload_rgba8(texture: "DeviceTexture", rgba, kind: int, stream: int | None = None) -> None:
+    """Host RGBA8 (h, w, 4) or (d, h, w, 4) into a device texture of any format: copy into an RGBA8
+    LINEAR staging texture, then mm_pack_texture (kind 0 weather, 1 curl, 2 low-res, 3 hi-res)."""
+    import torch
+
+    stream = current_stream() if stream is None else stream
+    d = texture.desc
+    stage = device_texture(d.width, d.height, d.depth, FMT_RGBA8, LAYOUT_LINEAR, "cuda")
+    stage.tensor.copy_(torch.from_numpy(rgba.reshape(-1)))
+    check(lib().mm_pack_texture(kind, ctypes.byref(stage.desc), ctypes.byref(d), stream), "mm_pack_texture")
+    torch.cuda.synchronize()
 
 
 def previous_camera(cam: CameraUBO) -> CameraUBO:

@@ -57,6 +57,7 @@ int check_texture(const mm_texture* t, int slot, int dims, bool need_pow2) {
     }
     size_t need = mm_texture_bytes(t->width, t->height, t->depth, t->format, t->layout);
     if (need == 0) return MM_E_FORMAT;
+    if (need > 0xffffffffull) return MM_E_DIM;  // the kernels address texels with 32-bit byte offsets
     if (t->bytes < need) return MM_E_SIZE;
     return MM_OK;
 }
@@ -128,6 +129,7 @@ const char* mm_strerror(int code) {
         case MM_E_DEVICE: return "HIP runtime error";
         case MM_E_SIZE: return "texture allocation too small";
         case MM_E_VERSION: return "struct size mismatch";
+        case MM_E_IO: return "file could not be opened or read";
     }
     return "unknown error";
 }
@@ -594,6 +596,90 @@ int mm_cloud_shadow(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm
     c.n = n;
     reset_error_state();
     return mm::launch_shadow(fset, c, view_of(*weather), view_of(*low_res), points, out, stream);
+}
+
+void mm_post_params_reference(mm_post_params* p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->size = sizeof(mm_post_params);
+    p->subtexel_bits = 8;
+    p->god_ray_samples = 8;                       // NUM_SAMPLES (god-ray.frag:31)
+    p->god_ray_density = 0.75f;                   // DENSITY
+    p->god_ray_decay = 0.99f;                     // DECAY
+    p->god_ray_exposure = 0.9f;                   // EXPOSURE
+    p->blur_scale = 1.5f;                         // radialBlur.frag:59
+    p->blur_gain = 1.1f;
+    p->exposure = 0.7f;                           // tonemap.frag:30
+    p->inv_gamma = 1.0f / 2.2f;
+    p->white_point = 50.2f;                       // tonemap.frag:29
+    p->vignette[0] = 0.1f;                        // tonemap.frag:34
+    p->vignette[1] = 0.05f;
+    p->vignette[2] = 0.13f;
+}
+
+int mm_post_process(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const float* image_rgba32f, int32_t width,
+                    int32_t height, int32_t row_pitch_px, float* god_alpha, float* out_hdr_rgba32f, uint8_t* out_rgba8,
+                    const mm_post_params* params, void* stream) {
+    if (!camera || !sun || !image_rgba32f || !god_alpha) return MM_E_NULL;
+    if (width <= 0 || height <= 0 || width > 16384 || height > 16384) return MM_E_DIM;
+    if (row_pitch_px < width || (int64_t)row_pitch_px * height > (1ll << 28)) return MM_E_PITCH;
+    mm_post_params ref;
+    if (!params) {
+        mm_post_params_reference(&ref);
+        params = &ref;
+    }
+    if (params->size != sizeof(mm_post_params)) return MM_E_VERSION;
+    const float f[] = {params->god_ray_density, params->god_ray_decay, params->god_ray_exposure, params->blur_scale,
+                       params->blur_gain, params->exposure, params->inv_gamma, params->white_point,
+                       params->vignette[0], params->vignette[1], params->vignette[2]};
+    if (!finite_n(f, (int)(sizeof(f) / sizeof(f[0])))) return MM_E_PARAM;
+    if (params->subtexel_bits < 1 || params->subtexel_bits > 8) return MM_E_PARAM;
+    if (params->god_ray_samples < 1 || params->god_ray_samples > 256) return MM_E_PARAM;
+    if (!finite_n(camera->view, 16) || !finite_n(camera->proj, 16) || !finite_n(sun->location, 4) ||
+        !finite_n(sun->direction, 4) || !finite_n(sun->color, 3) || !isfinite(sun->intensity))
+        return MM_E_PARAM;
+    mm::PostConsts c;
+    memset(&c, 0, sizeof(c));
+    // the sun in NDC: (proj * view) * location, then / w (god-ray.frag:47-49, radialBlur.frag:52-53)
+    const float* P = camera->proj;
+    const float* V = camera->view;
+    float M[16];
+    for (int col = 0; col < 4; col++)
+        for (int r = 0; r < 4; r++)
+            M[col * 4 + r] = ((P[0 * 4 + r] * V[col * 4 + 0] + P[1 * 4 + r] * V[col * 4 + 1]) +
+                              P[2 * 4 + r] * V[col * 4 + 2]) + P[3 * 4 + r] * V[col * 4 + 3];
+    float sp[4];
+    for (int r = 0; r < 4; r++)
+        sp[r] = ((M[0 * 4 + r] * sun->location[0] + M[1 * 4 + r] * sun->location[1]) + M[2 * 4 + r] * sun->location[2]) +
+                M[3 * 4 + r] * sun->location[3];
+    c.sunX = sp[0] / sp[3];
+    c.sunY = sp[1] / sp[3];
+    c.width = width;
+    c.height = height;
+    c.pitch = row_pitch_px;
+    c.bits = params->subtexel_bits;
+    c.samples = params->god_ray_samples;
+    c.below = sun->direction[1] < 0.0f ? 1 : 0;
+    c.sw = 1.0f / (float)params->god_ray_samples;
+    c.k = c.sw * params->god_ray_density;
+    c.decay = params->god_ray_decay;
+    c.godExposure = params->god_ray_exposure;
+    c.blurScale = params->blur_scale;
+    c.blurGain = params->blur_gain;
+    for (int k = 0; k < 3; k++) {
+        c.gc[k] = sun->color[k] * sun->intensity;
+        c.vignette[k] = params->vignette[k];
+    }
+    c.exposure = params->exposure;
+    c.invGamma = params->inv_gamma;
+    {
+        // 1 / uc2(white point) (tonemap.frag:20), the curve as the kernels evaluate it
+        const float x = params->white_point;
+        const float A = 0.15f, k05 = 0.1f * 0.5f, k004 = 0.2f * 0.02f, k06 = 0.2f * 0.3f, k0067 = 0.02f / 0.3f;
+        c.whitemap = 1.0f / (((x * (A * x + k05) + k004) / (x * (A * x + 0.5f) + k06)) - k0067);
+    }
+    reset_error_state();
+    return mm::launch_post(c, image_rgba32f, god_alpha, out_hdr_rgba32f, out_rgba8, stream);
 }
 
 struct mm_view_plan {

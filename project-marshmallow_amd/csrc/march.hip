@@ -61,6 +61,13 @@ __device__ __forceinline__ float lerpf(float a, float b, float t) {
     return a + t * (b - a);
 #endif
 }
+// a + t (b - a) with the product rounded: the scalar restatement's filter arithmetic.
+__device__ __forceinline__ float lerp_rounded(float a, float b, float t) { return a + t * (b - a); }
+template <bool FUSED>
+__device__ __forceinline__ float lerp_as(float a, float b, float t) {
+    if constexpr (FUSED) return lerpf(a, b, t);
+    else return lerp_rounded(a, b, t);
+}
 __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
     float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
     return t * t * (3.0f - 2.0f * t);
@@ -90,12 +97,21 @@ __device__ __forceinline__ float remapClamped01(float v, float a) {
 // ------------------------------------------------------------------ sampler
 // x = u * n - 0.5, optionally rounded to 1/256 (fixed-point sub-texel weights), i0 = floor(x) mod n
 // (n a power of two: two's-complement AND is the floor modulo), f = x - floor(x).
+// u * n is exact (n a power of two), so fma(u, n, -0.5) rounds like u * n - 0.5. With the 1/256 rounding,
+// k = floor(x * 256 + 0.5) (x * 256 exact) gives floor(k / 256) = k >> 8 and the fraction
+// (k & 255) / 256 exactly as the float steps do, for texel coordinates below 2^23 in magnitude
+// (8.4 M texels; the presets stay below 1e5).
 __device__ __forceinline__ void axis_p2(float u, int n, int q8, int& i0, float& f) {
-    float x = u * (float)n - 0.5f;
-    if (q8) x = floorf(x * 256.0f + 0.5f) * (1.0f / 256.0f);
-    float fl = floorf(x);
-    f = x - fl;
-    i0 = ((int)fl) & (n - 1);
+    const float x = __builtin_fmaf(u, (float)n, -0.5f);
+    if (q8) {
+        const int k = (int)floorf(__builtin_fmaf(x, 256.0f, 0.5f));
+        i0 = (k >> 8) & (n - 1);
+        f = (float)(k & 255) * (1.0f / 256.0f);
+    } else {
+        const float fl = floorf(x);
+        f = x - fl;
+        i0 = ((int)fl) & (n - 1);
+    }
 }
 
 __device__ __forceinline__ uint32_t ld_u32(const uint8_t* p) { return *reinterpret_cast<const uint32_t*>(p); }
@@ -157,16 +173,18 @@ __device__ __forceinline__ G2<BYTES> gather2(const TexView& tv, float u, float v
     int x1, y1;
     if (APRON) { x1 = x0 + 1; y1 = y0 + 1; }
     else { x1 = (x0 + 1) & (tv.w - 1); y1 = (y0 + 1) & (tv.h - 1); }
-    const uint8_t* r0 = tv.data + (size_t)(y0 * tv.stride) * BYTES;
-    const uint8_t* r1 = tv.data + (size_t)(y1 * tv.stride) * BYTES;
+    // 32-bit byte offsets from the uniform base (textures are below 4 GiB, checked at the boundary):
+    // the loads take the scalar-base + vector-offset form, with no 64-bit address arithmetic
+    // (row and slice products fit 24-bit operands: dims <= 8192, slices <= 1025^2 texels)
+    const uint32_t r0 = __umul24((uint32_t)y0, (uint32_t)tv.stride), r1 = __umul24((uint32_t)y1, (uint32_t)tv.stride);
     if constexpr (APRON) {
-        ld_pair<BYTES>(r0 + x0 * BYTES, g.t[0], g.t[1]);
-        ld_pair<BYTES>(r1 + x0 * BYTES, g.t[2], g.t[3]);
+        ld_pair<BYTES>(tv.data + (r0 + (uint32_t)x0) * BYTES, g.t[0], g.t[1]);
+        ld_pair<BYTES>(tv.data + (r1 + (uint32_t)x0) * BYTES, g.t[2], g.t[3]);
     } else {
-        g.t[0] = ld_texel<BYTES>(r0 + x0 * BYTES);
-        g.t[1] = ld_texel<BYTES>(r0 + x1 * BYTES);
-        g.t[2] = ld_texel<BYTES>(r1 + x0 * BYTES);
-        g.t[3] = ld_texel<BYTES>(r1 + x1 * BYTES);
+        g.t[0] = ld_texel<BYTES>(tv.data + (r0 + (uint32_t)x0) * BYTES);
+        g.t[1] = ld_texel<BYTES>(tv.data + (r0 + (uint32_t)x1) * BYTES);
+        g.t[2] = ld_texel<BYTES>(tv.data + (r1 + (uint32_t)x0) * BYTES);
+        g.t[3] = ld_texel<BYTES>(tv.data + (r1 + (uint32_t)x1) * BYTES);
     }
     return g;
 }
@@ -183,43 +201,44 @@ __device__ __forceinline__ G3<BYTES> gather3(const TexView& tv, float u, float v
     if (APRON) { x1 = x0 + 1; y1 = y0 + 1; z1 = z0 + 1; }
     else { x1 = (x0 + 1) & (n - 1); y1 = (y0 + 1) & (n - 1); z1 = (z0 + 1) & (n - 1); }
     const uint8_t* b = tv.data;
-    const size_t r00 = (size_t)(z0 * tv.sliceStride + y0 * tv.stride);
-    const size_t r10 = (size_t)(z0 * tv.sliceStride + y1 * tv.stride);
-    const size_t r01 = (size_t)(z1 * tv.sliceStride + y0 * tv.stride);
-    const size_t r11 = (size_t)(z1 * tv.sliceStride + y1 * tv.stride);
+    const uint32_t s0 = __umul24((uint32_t)z0, (uint32_t)tv.sliceStride), s1 = __umul24((uint32_t)z1, (uint32_t)tv.sliceStride);
+    const uint32_t q0 = __umul24((uint32_t)y0, (uint32_t)tv.stride), q1 = __umul24((uint32_t)y1, (uint32_t)tv.stride);
+    const uint32_t r00 = s0 + q0, r10 = s0 + q1, r01 = s1 + q0, r11 = s1 + q1;
+    const uint32_t ux0 = (uint32_t)x0, ux1 = (uint32_t)x1;
     if constexpr (APRON) {
-        ld_pair<BYTES>(b + (r00 + x0) * BYTES, g.t[0], g.t[1]);
-        ld_pair<BYTES>(b + (r10 + x0) * BYTES, g.t[2], g.t[3]);
-        ld_pair<BYTES>(b + (r01 + x0) * BYTES, g.t[4], g.t[5]);
-        ld_pair<BYTES>(b + (r11 + x0) * BYTES, g.t[6], g.t[7]);
+        ld_pair<BYTES>(b + (r00 + ux0) * BYTES, g.t[0], g.t[1]);
+        ld_pair<BYTES>(b + (r10 + ux0) * BYTES, g.t[2], g.t[3]);
+        ld_pair<BYTES>(b + (r01 + ux0) * BYTES, g.t[4], g.t[5]);
+        ld_pair<BYTES>(b + (r11 + ux0) * BYTES, g.t[6], g.t[7]);
     } else {
-        g.t[0] = ld_texel<BYTES>(b + (r00 + x0) * BYTES);
-        g.t[1] = ld_texel<BYTES>(b + (r00 + x1) * BYTES);
-        g.t[2] = ld_texel<BYTES>(b + (r10 + x0) * BYTES);
-        g.t[3] = ld_texel<BYTES>(b + (r10 + x1) * BYTES);
-        g.t[4] = ld_texel<BYTES>(b + (r01 + x0) * BYTES);
-        g.t[5] = ld_texel<BYTES>(b + (r01 + x1) * BYTES);
-        g.t[6] = ld_texel<BYTES>(b + (r11 + x0) * BYTES);
-        g.t[7] = ld_texel<BYTES>(b + (r11 + x1) * BYTES);
+        g.t[0] = ld_texel<BYTES>(b + (r00 + ux0) * BYTES);
+        g.t[1] = ld_texel<BYTES>(b + (r00 + ux1) * BYTES);
+        g.t[2] = ld_texel<BYTES>(b + (r10 + ux0) * BYTES);
+        g.t[3] = ld_texel<BYTES>(b + (r10 + ux1) * BYTES);
+        g.t[4] = ld_texel<BYTES>(b + (r01 + ux0) * BYTES);
+        g.t[5] = ld_texel<BYTES>(b + (r01 + ux1) * BYTES);
+        g.t[6] = ld_texel<BYTES>(b + (r11 + ux0) * BYTES);
+        g.t[7] = ld_texel<BYTES>(b + (r11 + ux1) * BYTES);
     }
     return g;
 }
 
-template <int BYTES, class F>
+// FUSED: the march's filter lerps (lerpf); otherwise the restatement's rounded product.
+template <bool FUSED = true, int BYTES, class F>
 __device__ __forceinline__ float filt2(const G2<BYTES>& g, F ch) {
-    float a = lerpf(ch(g.t[0]), ch(g.t[1]), g.fx);
-    float b = lerpf(ch(g.t[2]), ch(g.t[3]), g.fx);
-    return lerpf(a, b, g.fy);
+    float a = lerp_as<FUSED>(ch(g.t[0]), ch(g.t[1]), g.fx);
+    float b = lerp_as<FUSED>(ch(g.t[2]), ch(g.t[3]), g.fx);
+    return lerp_as<FUSED>(a, b, g.fy);
 }
-template <int BYTES, class F>
+template <bool FUSED = true, int BYTES, class F>
 __device__ __forceinline__ float filt3(const G3<BYTES>& g, F ch) {
-    float a0 = lerpf(ch(g.t[0]), ch(g.t[1]), g.fx);
-    float b0 = lerpf(ch(g.t[2]), ch(g.t[3]), g.fx);
-    float a1 = lerpf(ch(g.t[4]), ch(g.t[5]), g.fx);
-    float b1 = lerpf(ch(g.t[6]), ch(g.t[7]), g.fx);
-    float s0 = lerpf(a0, b0, g.fy);
-    float s1 = lerpf(a1, b1, g.fy);
-    return lerpf(s0, s1, g.fz);
+    float a0 = lerp_as<FUSED>(ch(g.t[0]), ch(g.t[1]), g.fx);
+    float b0 = lerp_as<FUSED>(ch(g.t[2]), ch(g.t[3]), g.fx);
+    float a1 = lerp_as<FUSED>(ch(g.t[4]), ch(g.t[5]), g.fx);
+    float b1 = lerp_as<FUSED>(ch(g.t[6]), ch(g.t[7]), g.fx);
+    float s0 = lerp_as<FUSED>(a0, b0, g.fy);
+    float s1 = lerp_as<FUSED>(a1, b1, g.fy);
+    return lerp_as<FUSED>(s0, s1, g.fz);
 }
 
 // Channel decoders.
@@ -253,17 +272,17 @@ struct Weather {
     float coverage, type;
 };
 
-template <int FSET>
+template <int FSET, bool FUSED = true>
 __device__ __forceinline__ Weather read_weather(const TexView& tv, float u, float v, int q8) {
     Weather w;
     if constexpr (FSET == mm::FSET_REF) {
         G2<4> g = gather2<4, false>(tv, u, v, q8);
-        w.coverage = filt2(g, Byte<0>()) * kInv255;
-        w.type = filt2(g, Byte<2>()) * kInv255;
+        w.coverage = filt2<FUSED>(g, Byte<0>()) * kInv255;
+        w.type = filt2<FUSED>(g, Byte<2>()) * kInv255;
     } else {
         G2<2> g = gather2<2, true>(tv, u, v, q8);
-        w.coverage = filt2(g, Byte<0>()) * kInv255;
-        w.type = filt2(g, Byte<1>()) * kInv255;
+        w.coverage = filt2<FUSED>(g, Byte<0>()) * kInv255;
+        w.type = filt2<FUSED>(g, Byte<1>()) * kInv255;
     }
     return w;
 }
@@ -309,21 +328,22 @@ __device__ __forceinline__ LowG<FSET> gather_low(const TexView& tv, V3 p, int q8
     r.g = gather3<4, FSET != mm::FSET_REF>(tv, p.x, p.y, p.z, q8);
     return r;
 }
-template <int FSET>
+template <int FSET, bool FUSED = true>
 __device__ __forceinline__ float low_shape(const LowG<FSET>& l, const TexView& tv) {
-    if constexpr (FSET == mm::FSET_PREC) return filt3(l.g, Fp8<0>()) * tv.scale[0];
-    else return filt3(l.g, Byte<0>()) * kInv255;
+    if constexpr (FSET == mm::FSET_PREC) return filt3<FUSED>(l.g, Fp8<0>()) * tv.scale[0];
+    else return filt3<FUSED>(l.g, Byte<0>()) * kInv255;
 }
-template <int FSET>
+template <int FSET, bool FUSED = true>
 __device__ __forceinline__ float low_erosion(const LowG<FSET>& l, const TexView& tv) {
     if constexpr (FSET == mm::FSET_REF) {
-        float g = filt3(l.g, Byte<1>()) * kInv255, b = filt3(l.g, Byte<2>()) * kInv255, a = filt3(l.g, Byte<3>()) * kInv255;
+        float g = filt3<FUSED>(l.g, Byte<1>()) * kInv255, b = filt3<FUSED>(l.g, Byte<2>()) * kInv255,
+              a = filt3<FUSED>(l.g, Byte<3>()) * kInv255;
         return ((0.625f * g) + (0.25f * b)) + (0.125f * a);
     } else if constexpr (FSET == mm::FSET_FAST) {
-        return filt3(l.g, Hi16()) * kInv2040;
+        return filt3<FUSED>(l.g, Hi16()) * kInv2040;
     } else {
-        float g = filt3(l.g, Fp8<1>()) * tv.scale[1], b = filt3(l.g, Fp8<2>()) * tv.scale[2],
-              a = filt3(l.g, Fp8<3>()) * tv.scale[3];
+        float g = filt3<FUSED>(l.g, Fp8<1>()) * tv.scale[1], b = filt3<FUSED>(l.g, Fp8<2>()) * tv.scale[2],
+              a = filt3<FUSED>(l.g, Fp8<3>()) * tv.scale[3];
         return ((0.625f * g) + (0.25f * b)) + (0.125f * a);
     }
 }
@@ -811,17 +831,23 @@ __global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const Ma
 // performs exactly the shader's operations in the shader's order.
 constexpr int kQueueCap = 128;
 
-struct ConeQueue {
+// light-cone modes: MM_CONE_PLAIN, MM_CONE_SURROGATE, and the training-sample recorder
+enum ConeMode { CONE_PLAIN = MM_CONE_PLAIN, CONE_SURROGATE = MM_CONE_SURROGATE, CONE_COLLECT = 2 };
+
+// The plain queue is 4 KB per wave (LDS then allows the full 8 waves per SIMD); the surrogate and
+// recorder modes also keep each job's surrogate features and a compaction list.
+struct ConeQueuePlain {
     float x[kQueueCap], y[kQueueCap], z[kQueueCap], step[kQueueCap];
-    float x0[kQueueCap], w[kQueueCap], blend[kQueueCap], res[kQueueCap];
-    float rh[kQueueCap], lo[kQueueCap], dens[kQueueCap];
+    float x0[kQueueCap], w[kQueueCap], res[kQueueCap];
     int owner[kQueueCap];
+};
+struct ConeQueueFull : ConeQueuePlain {
+    float rh[kQueueCap], lo[kQueueCap], dens[kQueueCap];
     int fallback[64];
     float dal[64];
 };
-
-// light-cone modes: MM_CONE_PLAIN, MM_CONE_SURROGATE, and the training-sample recorder
-enum ConeMode { CONE_PLAIN = MM_CONE_PLAIN, CONE_SURROGATE = MM_CONE_SURROGATE, CONE_COLLECT = 2 };
+template <int MODE> struct ConeQueueOf { typedef ConeQueueFull type; };
+template <> struct ConeQueueOf<CONE_PLAIN> { typedef ConeQueuePlain type; };
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
@@ -830,16 +856,19 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
 
 // Evaluates the first min(qn, 64) queued cone jobs, one per lane, leaving each job's term
 // x0 * Beer(D) in q.res (and, in the recorder mode, appending training rows).
-template <int FSET, bool COUNT, int MODE>
-__device__ __forceinline__ void run_batch(const MarchConsts& c, const MarchTextures& tx, ConeQueue& q, const int nb,
-                                          int& nd, int& nh, int& nj, int& nc) {
+// myBlend is the calling lane's own Beer blend; a job's blend is its owner's, fetched by a lane
+// shuffle rather than stored in the queue.
+template <int FSET, bool COUNT, int MODE, class Q>
+__device__ __forceinline__ void run_batch(const MarchConsts& c, const MarchTextures& tx, Q& q, const int nb,
+                                          const float myBlend, int& nd, int& nh, int& nj, int& nc) {
     const int lane = threadIdx.x;
+    const float blend = __shfl(myBlend, q.owner[lane]);  // every lane takes part; slots >= nb are unused
     if (COUNT) nj += nb;
     if constexpr (MODE == CONE_PLAIN) {
         if (COUNT) nc += nb;
         if (lane < nb) {
             float dal = cone_dal<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], nd, nh);
-            q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
+            q.res[lane] = q.x0[lane] * beers_of(dal, blend);
         }
     } else if constexpr (MODE == CONE_COLLECT) {
         if (COUNT) nc += nb;
@@ -850,7 +879,7 @@ __device__ __forceinline__ void run_batch(const MarchConsts& c, const MarchTextu
         if (lane < nb) {
             const V3 pos = mk(q.x[lane], q.y[lane], q.z[lane]);
             dal = cone_dal<FSET, COUNT>(c, tx, pos, q.step[lane], nd, nh);
-            q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
+            q.res[lane] = q.x0[lane] * beers_of(dal, blend);
             const uint32_t hsh = mix32(__float_as_uint(pos.x) ^ mix32(__float_as_uint(pos.y) ^
                                        mix32(__float_as_uint(pos.z) ^ c.collectSeed)));
             keep = (hsh % c.collectKeep) == 0u;
@@ -895,21 +924,24 @@ __device__ __forceinline__ void run_batch(const MarchConsts& c, const MarchTextu
         __syncthreads();
         if (lane < nb) {
             const float dal = fb ? q.dal[lane] : fmaxf(0.0f, -o.o0);
-            q.res[lane] = q.x0[lane] * beers_of(dal, q.blend[lane]);
+            q.res[lane] = q.x0[lane] * beers_of(dal, blend);
         }
     }
     __syncthreads();
 }
 
 // Drops the first nb (= 64 when qn > 64) jobs of the queue, moving the remainder to the front.
-__device__ __forceinline__ void queue_shift(ConeQueue& q, const int nb, const int rem) {
+template <class Q>
+__device__ __forceinline__ void queue_shift(Q& q, const int nb, const int rem) {
     const int lane = threadIdx.x;
     __syncthreads();
     if (lane < rem) {  // rem < 64 and only when nb == 64: source and destination slots are disjoint
         const int s = nb + lane;
         q.x[lane] = q.x[s]; q.y[lane] = q.y[s]; q.z[lane] = q.z[s]; q.step[lane] = q.step[s];
-        q.x0[lane] = q.x0[s]; q.w[lane] = q.w[s]; q.blend[lane] = q.blend[s]; q.owner[lane] = q.owner[s];
-        q.rh[lane] = q.rh[s]; q.lo[lane] = q.lo[s]; q.dens[lane] = q.dens[s];
+        q.x0[lane] = q.x0[s]; q.w[lane] = q.w[s]; q.owner[lane] = q.owner[s];
+        if constexpr (sizeof(Q) == sizeof(ConeQueueFull)) {
+            q.rh[lane] = q.rh[s]; q.lo[lane] = q.lo[s]; q.dens[lane] = q.dens[s];
+        }
     }
     __syncthreads();
 }
@@ -920,7 +952,7 @@ __device__ __forceinline__ void queue_shift(ConeQueue& q, const int nb, const in
 template <int FSET, bool COUNT, int MODE>
 __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchTextures& tx, const int tile,
                                            float* __restrict__ out, unsigned long long* __restrict__ counts) {
-    __shared__ ConeQueue q;
+    __shared__ typename ConeQueueOf<MODE>::type q;
     const int lane = threadIdx.x;
     const int gi = (tile % c.tilesX) * (1 << c.tileShift) + (lane & ((1 << c.tileShift) - 1));
     const int gj = (tile / c.tilesX) * (64 >> c.tileShift) + (lane >> c.tileShift);
@@ -951,15 +983,17 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
                 const int slot = qn + __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
                 q.x[slot] = jp.x; q.y[slot] = jp.y; q.z[slot] = jp.z; q.step[slot] = js;
-                q.x0[slot] = jx0; q.w[slot] = jw; q.blend[slot] = r.beerBlend; q.owner[slot] = lane;
-                q.rh[slot] = jrh; q.lo[slot] = jlo; q.dens[slot] = jd;
+                q.x0[slot] = jx0; q.w[slot] = jw; q.owner[slot] = lane;
+                if constexpr (MODE != CONE_PLAIN) {
+                    q.rh[slot] = jrh; q.lo[slot] = jlo; q.dens[slot] = jd;
+                }
             }
             qn += __popcll(mask);
         }
         if (qn == 0) break;
         __syncthreads();
         const int nb = qn < 64 ? qn : 64;
-        run_batch<FSET, COUNT, MODE>(c, tx, q, nb, nd, nh, nj, nc);
+        run_batch<FSET, COUNT, MODE>(c, tx, q, nb, valid ? r.beerBlend : 0.0f, nd, nh, nj, nc);
         for (int s = 0; s < nb; s++)
             if (q.owner[s] == lane) transmittance = mixf(transmittance, q.res[s], q.w[s]);
         queue_shift(q, nb, qn - nb);
@@ -1002,7 +1036,8 @@ void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* 
 // ------------------------------------------------------------------ cloud shadows (model.frag)
 // One work-item per shaded point: the mesh pass's six-step low-res shadow march (model.frag:240-270)
 // with that shader's own constants and cloud functions (model.frag:52-129). Remaps divide exactly
-// like the shader, so the REF texture set reproduces the scalar restatement up to pow's last bit.
+// like the shader and the filters round their products like the scalar restatement (the march
+// fuses them), so the REF texture set reproduces the restatement bit for bit.
 __device__ __forceinline__ float sh_remap(float v, float a, float b, float c, float d) {
     return c + ((__fdiv_rn(v - a, b - a)) * (d - c));
 }
@@ -1022,18 +1057,18 @@ template <int FSET>
 __device__ float sh_cloud_test(const mm::ShadowConsts& c, const TexView& wt, const TexView& lt, V3 pos, float h,
                                V3 center) {
     const V3 proj = (c.projRadius * normalize3(pos - center)) + center;
-    const Weather w = read_weather<FSET>(wt, c.weatherScale * (proj.x - c.camPos[0]),
-                                         c.weatherScale * (proj.z - c.camPos[2]), c.subtexelBits);
+    const Weather w = read_weather<FSET, false>(wt, c.weatherScale * (proj.x - c.camPos[0]),
+                                                c.weatherScale * (proj.z - c.camPos[2]), c.subtexelBits);
     const float layer = sh_layer(h, w.type);
     const LowG<FSET> lg = gather_low<FSET>(lt, pos * c.lowScale, c.subtexelBits);
-    float density = layer * sh_remap_clamped(low_shape<FSET>(lg, lt), 0.3f, 1.0f, 0.0f, 1.0f);
+    float density = layer * sh_remap_clamped(low_shape<FSET, false>(lg, lt), 0.3f, 1.0f, 0.0f, 1.0f);
     if (density < 0.0001f) return 0.0f;
     const float e = clampf(sh_remap(fminf(0.85f, w.coverage), 0.7f, 0.8f, 1.0f, 0.6f), 0.6f, 1.0f);
     // pow evaluated as exp2(e * log2(h)) in double and rounded once (the scalar restatement uses the
     // same formula): the shadow is not on the frame's critical path, and the single rounding keeps
     // the 1e-4 early out and the erosion remap from amplifying last-bit differences
     const float coverage = (float)exp2((double)e * log2((double)h));
-    const float erosion = sh_remap_clamped(low_erosion<FSET>(lg, lt), coverage, 1.0f, 0.0f, 1.0f);
+    const float erosion = sh_remap_clamped(low_erosion<FSET, false>(lg, lt), coverage, 1.0f, 0.0f, 1.0f);
     return sh_remap_clamped(density, erosion, 1.0f, 0.0f, 1.0f);
 }
 

@@ -89,6 +89,18 @@ struct ShadowConsts {
 int launch_shadow(int fset, const ShadowConsts& c, const TexView& weather, const TexView& low, const float* pts,
                   float* out, void* stream);
 
+// Constants of one post-chain pass (god-ray.frag, radialBlur.frag, tonemap.frag), derived on the host.
+struct PostConsts {
+    int width, height, pitch, bits, samples, below;
+    float sunX, sunY;             // the sun in NDC: ((proj * view) * location).xy / w
+    float k;                      // (1 / samples) * density
+    float sw;                     // 1 / samples
+    float decay, godExposure, blurScale, blurGain;
+    float gc[3];                  // sun.color.rgb * sun.intensity
+    float exposure, invGamma, whitemap, vignette[3];
+};
+int launch_post(const PostConsts& c, const float* image, float* godAlpha, float* hdr, uint8_t* rgba8, void* stream);
+
 // One view of a multi-view launch (device array, built on the host).
 struct ViewDesc {
     MarchConsts c;

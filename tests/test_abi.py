@@ -130,6 +130,8 @@ def test_march_rejects_bad_textures(mm, cpu_only):
     assert _march(mm, _noise(mm, hi_res=_FakeTex(mm, 32, 32, 32, mm.FMT_RGBA8, 0, nbytes=100))) == mm.MM_E_SIZE
     assert _march(mm, _noise(mm, hi_res=_FakeTex(mm, 32, 32, 32, mm.FMT_E16, mm.LAYOUT_APRON))) == mm.MM_E_FORMAT
     assert _march(mm, _noise(mm, curl=_FakeTex(mm, 128, 128, 1, mm.FMT_R8E16, 0))) == mm.MM_E_FORMAT
+    # a volume of 4 GiB or more (1024^3 RGBA8 = exactly 4 GiB): the kernels use 32-bit byte offsets
+    assert _march(mm, _noise(mm, low_res=_FakeTex(mm, 1024, 1024, 1024, mm.FMT_RGBA8, 0))) == mm.MM_E_DIM
     n = _noise(mm)
     n.weather.data = None
     assert _march(mm, n) == mm.MM_E_NULL
