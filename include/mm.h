@@ -288,8 +288,9 @@ int mm_cloud_shadow(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm
  *   radial blur (radialBlur.frag:36-63) rgb = sun.color * intensity * mean(10 alpha taps * 1.1) + 0.5 rgb
  *   tonemap     (tonemap.frag:11-34)    Uncharted-2 curve, white point, gamma 1/2.2, vignette
  * image: device RGBA32F cloud image (the march or reprojection output), row_pitch_px pixels per row
- * (the same pitch for every image here). god_alpha: device float scratch, one per pixel (framebuffer
- * 1's alpha). out_hdr_rgba32f (optional): the radial-blur output (framebuffer 2 before meshes).
+ * (the same pitch for every image here). god_alpha: device float scratch of two planes of
+ * row_pitch_px * height floats (the image's alpha, then framebuffer 1's alpha, which the god-ray
+ * pass produces). out_hdr_rgba32f (optional): the radial-blur output (framebuffer 2 before meshes).
  * out_rgba8 (optional): the tonemapped frame as UNORM8 R, G, B, A = 255 (the swapchain is
  * B8G8R8A8_UNORM, VulkanApplication.cpp:1439; the channel order here is RGBA). */
 typedef struct mm_post_params {

@@ -100,6 +100,7 @@ int orc_cloud_shadow(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_s
 /* ---- post chain (god-ray.frag, radialBlur.frag, tonemap.frag) ---- */
 void orc_post_sun_ndc(const mm_camera_ubo* cam, const mm_sun_ubo* sun, float out_xy[2]); /* (proj*view)*location / w */
 float orc_post_uc2(float x);                                                         /* tonemap.frag:11-14 */
+float orc_post_pow(float x, float y);                                 /* exp2(y log2 x) as float series */
 int orc_post_process(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const float* image, int width, int height,
                      int pitch, float* god_alpha, float* hdr, uint8_t* rgba8, const mm_post_params* p);
 

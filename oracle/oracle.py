@@ -75,6 +75,7 @@ class Oracle:
             "orc_shadow_layer_density": (f, [f, f]),
             "orc_post_sun_ndc": (None, [vp, vp, vp]),
             "orc_post_uc2": (f, [f]),
+            "orc_post_pow": (f, [f, f]),
             "orc_post_process": (i, [vp, vp, vp, i, i, i, vp, vp, vp, vp]),
         }
         for name, (res, args) in sigs.items():

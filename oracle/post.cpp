@@ -16,8 +16,9 @@
 //     [0, n - 1] (the float floor is clamped to [-1, n] before the integer conversion), weights
 //     applied as a + f (b - a) with the product rounded;
 //   * proj * view * location is (proj * view) * location, sums in index order;
-//   * pow(x, y) = exp2(y log2 x) in double, rounded once; UNORM8 = floor(clamp(v, 0, 1) * 255 + 0.5),
-//     NaN -> 0;
+//   * pow(x, y) = exp2(y log2 x) with both written out as float series (pow_f below; GLSL defines pow
+//     through exp2/log2 and leaves their last bits to the implementation); UNORM8 =
+//     floor(clamp(v, 0, 1) * 255 + 0.5), NaN -> 0;
 //   * sub-texel precision of 1..8 bits (8 is the hardware's): a pixel-centre tap then returns the
 //     texel itself, so the tonemap's read of framebuffer 2 is the radial-blur value of that pixel.
 #include <math.h>
@@ -56,7 +57,58 @@ inline float sample_ch(const float* img, int w, int h, int stride, int step, flo
     return lerp_r(a, b, fy);
 }
 
-inline float pow_d(float x, float y) { return (float)exp2((double)y * log2((double)x)); }
+// pow(x, y) = exp2(y log2 x), both written out as float operations (the HIP kernel evaluates the
+// same operations in the same order). log2: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log2 m by the
+// atanh series in s = (m - 1) / (m + 1) to s^9; exp2: t = n + f, 2^f = sqrt(2) 2^(f - 1/2) by the
+// exponential series in g = (f - 1/2) ln 2 to g^7, scaled by 2^n exactly. exp2 within 3 ulp, log2
+// within 1e-6 absolute; pow within 1e-6 relative over the tonemap's inputs (the float rounding of
+// y log2 x dominates).
+inline float log2_series(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : NAN;
+    if (x == INFINITY) return INFINITY;
+    int e = 0;
+    if (x < 1.17549435e-38f) {  // subnormal: scale into the normal range
+        x = x * 16777216.0f;
+        e = -24;
+    }
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    e += (int)((b >> 23) & 255u) - 127;
+    b = (b & 0x007fffffu) | 0x3f800000u;
+    float m;
+    memcpy(&m, &b, 4);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p = 0.320598898f;
+    p = p * s2 + 0.412198583f;
+    p = p * s2 + 0.577078016f;
+    p = p * s2 + 0.961796694f;
+    p = p * s2 + 2.88539008f;
+    return (float)e + s * p;
+}
+
+inline float exp2_series(float t) {
+    if (t != t) return t;
+    if (t >= 128.0f) return INFINITY;
+    if (t < -150.0f) return 0.0f;
+    const float n = floorf(t);
+    const float g = ((t - n) - 0.5f) * 0.693147181f;
+    float p = 1.98412698e-4f;
+    p = p * g + 1.38888889e-3f;
+    p = p * g + 8.33333333e-3f;
+    p = p * g + 4.16666667e-2f;
+    p = p * g + 0.166666667f;
+    p = p * g + 0.5f;
+    p = p * g + 1.0f;
+    p = p * g + 1.0f;
+    return ldexpf(p * 1.41421356f, (int)n);
+}
+
+inline float pow_f(float x, float y) { return exp2_series(y * log2_series(x)); }
 
 inline float uc2(float x) {
     const float A = 0.15f, k05 = 0.1f * 0.5f, k004 = 0.2f * 0.02f, k06 = 0.2f * 0.3f, k0067 = 0.02f / 0.3f;
@@ -87,6 +139,7 @@ extern "C" void orc_post_sun_ndc(const mm_camera_ubo* cam, const mm_sun_ubo* sun
 }
 
 extern "C" float orc_post_uc2(float x) { return uc2(x); }
+extern "C" float orc_post_pow(float x, float y) { return pow_f(x, y); }
 
 extern "C" int orc_post_process(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const float* image, int width,
                                 int height, int pitch, float* god_alpha, float* hdr, uint8_t* rgba8,
@@ -166,7 +219,7 @@ extern "C" int orc_post_process(const mm_camera_ubo* cam, const mm_sun_ubo* sun,
                     // the framebuffer is sampled at the pixel centre: the radial-blur value itself
                     float t = uc2(p->exposure * o[c]);
                     t = t * whitemap;
-                    t = pow_d(t, p->inv_gamma);
+                    t = pow_f(t, p->inv_gamma);
                     t = t * (1.0f - vig) + p->vignette[c] * vig;
                     q[c] = unorm8(t);
                 }

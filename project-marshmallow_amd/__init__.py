@@ -406,8 +406,8 @@ def post_params(**overrides) -> PostParams:
 def post_process(cam: CameraUBO, sun: SunUBO, image, width: int, height: int, god_alpha, hdr=None, rgba8=None,
                  pitch: int | None = None, params: PostParams | None = None, stream: int | None = None) -> None:
     """God rays -> radial blur -> tonemap of a device RGBA32F cloud image (mm_post_process).
-    god_alpha: device float32 scratch [pitch * height]; hdr: optional float32 [pitch * height * 4];
-    rgba8: optional uint8 [pitch * height * 4]."""
+    god_alpha: device float32 scratch [2 * pitch * height] (plane 1 = framebuffer 1's alpha);
+    hdr: optional float32 [pitch * height * 4]; rgba8: optional uint8 [pitch * height * 4]."""
     stream = current_stream() if stream is None else stream
     pitch = width if pitch is None else pitch
     pp = ctypes.byref(params) if params is not None else None

@@ -380,6 +380,10 @@ template <int FSET, bool COUNT>
 __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTextures& tx, V3 pos, float h, V3 center,
                                             int& nd) {
     if (COUNT) nd++;
+    // every layer profile is 0 at h = 0 and below 1e-7 for h >= 0.9 (the cumulus top; stratocumulus
+    // and stratus end at 0.7 and 0.3), for any cloud type in [0, 1]: the density test below would
+    // return 0 there whatever the textures hold, so the weather and low-res reads are skipped
+    if (!(h > 0.0f && h < 0.9f)) return 0.0f;
     const int q8 = c.subtexelBits;
     V3 proj = proj_shell(c, pos, center);
     Weather w = read_weather<FSET>(tx.weather, c.weatherScale * (proj.x - c.camPos[0]),

@@ -99,7 +99,7 @@ struct PostConsts {
     float gc[3];                  // sun.color.rgb * sun.intensity
     float exposure, invGamma, whitemap, vignette[3];
 };
-int launch_post(const PostConsts& c, const float* image, float* godAlpha, float* hdr, uint8_t* rgba8, void* stream);
+int launch_post(const PostConsts& c, const float* image, float* scratch, float* hdr, uint8_t* rgba8, void* stream);
 
 // One view of a multi-view launch (device array, built on the host).
 struct ViewDesc {

@@ -1,15 +1,17 @@
 // Post chain for gfx950: the HIP counterpart of the offscreen passes that consume the cloud image,
 // god-ray.frag:41-76 -> radialBlur.frag:36-63 -> tonemap.frag:11-34 (VulkanApplication.cpp:938-966).
 //
-// Two launches of 16x16-pixel blocks (four wave64 rows of 16x4):
-//   god_ray_kernel      one work-item per pixel: 9 bilinear alpha taps of the cloud image (centre +
-//                       8 toward the sun), writes the alpha plane of framebuffer 1 (4 B/pixel);
+// Three launches of 16x16-pixel blocks (four wave64 rows of 16x4):
+//   alpha_plane_kernel  the cloud image's alpha as a plane of floats (16 B read, 4 B written);
+//   god_ray_kernel      one work-item per pixel: 9 bilinear taps of that plane (centre + 8 toward
+//                       the sun), writes the alpha plane of framebuffer 1 (4 B/pixel);
 //   blur_tonemap_kernel one work-item per pixel: bilinear centre tap of the image's RGB, 10 bilinear
 //                       taps of that alpha plane toward the sun, the radial-blur colour, then the
 //                       Uncharted-2 tonemap + vignette, written as RGBA8 (and optionally the HDR
 //                       radial-blur value).
-// The taps of neighbouring pixels overlap, so after the first touch they come from L2: both passes
-// are bound by HBM at 16 B read + 4 B written (god rays) and 16 + 4 B read + 4 (or 20) B written.
+// The taps of neighbouring pixels overlap, so after the first touch they come from L2: the passes
+// are bound by HBM at 16 B read + 4 B written (alpha plane), 4 B read + 4 B written (god rays) and
+// 16 + 4 B read + 4 (or 20) B written (radial blur + tonemap).
 // Arithmetic follows the shaders' evaluation order with IEEE division/sqrt and no contraction; the
 // sampler and the other choices GLSL leaves open are the scalar restatement's, so the result is the
 // same bit for bit.
@@ -72,12 +74,70 @@ __device__ __forceinline__ float uc2(float x) {
     return __fdiv_rn(x * (A * x + k05) + k004, x * (A * x + 0.5f) + k06) - k0067;
 }
 
+// pow(x, y) = exp2(y log2 x), both written out as float operations (the scalar restatement
+// evaluates the same operations in the same order). log2: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// log2 m by the atanh series in s = (m - 1) / (m + 1) to s^9; exp2: t = n + f,
+// 2^f = sqrt(2) 2^(f - 1/2) by the exponential series in g = (f - 1/2) ln 2 to g^7, scaled by 2^n
+// exactly. exp2 within 3 ulp, log2 within 1e-6 absolute; pow within 1e-6 relative over the
+// tonemap's inputs (the float rounding of y log2 x dominates); the result is quantised to 8 bits.
+__device__ __forceinline__ float log2_series(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : __int_as_float(0x7fc00000);
+    if (x == INFINITY) return INFINITY;
+    int e = 0;
+    if (x < 1.17549435e-38f) {  // subnormal: scale into the normal range
+        x = x * 16777216.0f;
+        e = -24;
+    }
+    uint32_t b = __float_as_uint(x);
+    e += (int)((b >> 23) & 255u) - 127;
+    float m = __uint_as_float((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    const float s = __fdiv_rn(m - 1.0f, m + 1.0f);
+    const float s2 = s * s;
+    float p = 0.320598898f;
+    p = p * s2 + 0.412198583f;
+    p = p * s2 + 0.577078016f;
+    p = p * s2 + 0.961796694f;
+    p = p * s2 + 2.88539008f;
+    return (float)e + s * p;
+}
+
+__device__ __forceinline__ float exp2_series(float t) {
+    if (t != t) return t;
+    if (t >= 128.0f) return INFINITY;
+    if (t < -150.0f) return 0.0f;
+    const float n = floorf(t);
+    const float g = ((t - n) - 0.5f) * 0.693147181f;
+    float p = 1.98412698e-4f;
+    p = p * g + 1.38888889e-3f;
+    p = p * g + 8.33333333e-3f;
+    p = p * g + 4.16666667e-2f;
+    p = p * g + 0.166666667f;
+    p = p * g + 0.5f;
+    p = p * g + 1.0f;
+    p = p * g + 1.0f;
+    return __builtin_amdgcn_ldexpf(p * 1.41421356f, (int)n);
+}
+
 __device__ __forceinline__ uint32_t unorm8(float v) {
     if (!(v == v)) return 0u;
     return (uint32_t)floorf(clampf(v, 0.0f, 1.0f) * 255.0f + 0.5f);
 }
 
-__global__ __launch_bounds__(256) void god_ray_kernel(const mm::PostConsts c, const float* __restrict__ img,
+// The image's alpha as a plane of its own: the god-ray taps then read 4-byte texels (32 per
+// 128-byte line instead of 8 RGBA32F ones), which is what keeps their far-reaching taps in cache.
+__global__ __launch_bounds__(256) void alpha_plane_kernel(const mm::PostConsts c, const float4* __restrict__ img,
+                                                          float* __restrict__ plane) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= c.width || y >= c.height) return;
+    const uint32_t i = (uint32_t)y * (uint32_t)c.pitch + (uint32_t)x;
+    plane[i] = img[i].w;
+}
+
+__global__ __launch_bounds__(256) void god_ray_kernel(const mm::PostConsts c, const float* __restrict__ a0,
                                                       float* __restrict__ alpha) {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= c.width || y >= c.height) return;
@@ -86,12 +146,12 @@ __global__ __launch_bounds__(256) void god_ray_kernel(const mm::PostConsts c, co
         const float u = __fdiv_rn((float)x + 0.5f, (float)c.width), v = __fdiv_rn((float)y + 0.5f, (float)c.height);
         float cx = u * 2.0f - 1.0f, cy = v * 2.0f - 1.0f;
         const float dx = (cx - c.sunX) * c.k, dy = (cy - c.sunY) * c.k;
-        float acc = tap<4>(img, 3, c, u, v) * 0.5f;
+        float acc = tap<1>(a0, 0, c, u, v) * 0.5f;
         float decay = 1.0f;
         for (int i = 0; i < c.samples; i++) {
             cx = cx - dx;
             cy = cy - dy;
-            float s = tap<4>(img, 3, c, cx * 0.5f + 0.5f, cy * 0.5f + 0.5f) * 0.5f;
+            float s = tap<1>(a0, 0, c, cx * 0.5f + 0.5f, cy * 0.5f + 0.5f) * 0.5f;
             s = s * (c.sw * decay);
             acc = acc + s;
             decay = decay * c.decay;
@@ -137,7 +197,7 @@ __global__ __launch_bounds__(256) void blur_tonemap_kernel(const mm::PostConsts 
             // framebuffer 2 sampled at the pixel centre returns this pixel's radial-blur value
             float t = uc2(c.exposure * o[k]);
             t = t * c.whitemap;
-            t = (float)exp2((double)c.invGamma * log2((double)t));  // pow, evaluated once in double
+            t = exp2_series(c.invGamma * log2_series(t));  // pow (tonemap.frag:22)
             t = t * (1.0f - vig) + c.vignette[k] * vig;
             q |= unorm8(t) << (8 * k);
         }
@@ -149,11 +209,15 @@ __global__ __launch_bounds__(256) void blur_tonemap_kernel(const mm::PostConsts 
 
 namespace mm {
 
-int launch_post(const PostConsts& c, const float* image, float* godAlpha, float* hdr, uint8_t* rgba8, void* stream) {
+int launch_post(const PostConsts& c, const float* image, float* scratch, float* hdr, uint8_t* rgba8, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     dim3 grid((unsigned)((c.width + 15) / 16), (unsigned)((c.height + 15) / 16)), block(256);
-    hipLaunchKernelGGL(god_ray_kernel, grid, block, 0, s, c, image, godAlpha);
-    hipLaunchKernelGGL(blur_tonemap_kernel, grid, block, 0, s, c, (const float4*)image, (const float*)godAlpha,
+    // scratch: plane 0 = the image's alpha, plane 1 = framebuffer 1's alpha (the god-ray output)
+    float* a0 = scratch;
+    float* a1 = scratch + (size_t)c.pitch * c.height;
+    if (!c.below) hipLaunchKernelGGL(alpha_plane_kernel, grid, block, 0, s, c, (const float4*)image, a0);
+    hipLaunchKernelGGL(god_ray_kernel, grid, block, 0, s, c, (const float*)a0, a1);
+    hipLaunchKernelGGL(blur_tonemap_kernel, grid, block, 0, s, c, (const float4*)image, (const float*)a1,
                        (float4*)hdr, (uint32_t*)rgba8);
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }

@@ -15,12 +15,12 @@ def _run(mm, cam, sun, img, params, pitch=None):
     src = torch.zeros((H, pitch, 4), dtype=torch.float32)
     src[:, :W] = torch.from_numpy(img)
     src = src.cuda()
-    alpha = torch.full((H, pitch), -7.0, dtype=torch.float32, device="cuda")
+    alpha = torch.full((2, H, pitch), -7.0, dtype=torch.float32, device="cuda")
     hdr = torch.full((H, pitch, 4), -7.0, dtype=torch.float32, device="cuda")
     out = torch.zeros((H, pitch, 4), dtype=torch.uint8, device="cuda")
     mm.post_process(cam, sun, src, W, H, alpha, hdr, out, pitch=pitch, params=params)
     torch.cuda.synchronize()
-    return alpha.cpu().numpy()[:, :W], hdr.cpu().numpy()[:, :W], out.cpu().numpy()[:, :W]
+    return alpha[1].cpu().numpy()[:, :W], hdr.cpu().numpy()[:, :W], out.cpu().numpy()[:, :W]
 
 
 @pytest.mark.parametrize("W,H,pitch", [(160, 90, None), (37, 23, 48), (1920, 1080, None)])
@@ -81,7 +81,7 @@ def test_post_optional_outputs_and_errors(mm, gpu):
 
     cam, sun = _ubos(mm)
     img = torch.rand((20, 30, 4), dtype=torch.float32, device="cuda")
-    alpha = torch.zeros((20, 30), dtype=torch.float32, device="cuda")
+    alpha = torch.zeros((2, 20, 30), dtype=torch.float32, device="cuda")
     out = torch.zeros((20, 30, 4), dtype=torch.uint8, device="cuda")
     mm.post_process(cam, sun, img, 30, 20, alpha, None, out)            # no HDR output
     hdr = torch.zeros((20, 30, 4), dtype=torch.float32, device="cuda")

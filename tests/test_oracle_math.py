@@ -46,6 +46,17 @@ def test_cloud_layer_density_known_answers(orc, h, t, expect):
     assert orc.L.orc_layer_density(h, t) == pytest.approx(expect, abs=2e-6)
 
 
+def test_cloud_layer_density_vanishes_outside_the_profiles(orc):
+    # the march skips its texture reads where every profile is 0 (h = 0 and h >= 0.9, the cumulus
+    # top), which is only exact if the layer density is 0 there for every cloud type
+    hs = np.arange(np.float32(0.9).view(np.int32), np.float32(1.0).view(np.int32) + 1, 4099).view(np.float32)
+    for t in np.linspace(0.0, 1.0, 33, dtype=np.float32):
+        assert orc.L.orc_layer_density(0.0, float(t)) == 0.0
+        assert max(orc.L.orc_layer_density(float(h), float(t)) for h in hs) == 0.0
+    # and not beyond: just below the cumulus top a cumulus-type cloud is still dense
+    assert orc.L.orc_layer_density(0.89, 1.0) > 1e-3
+
+
 @pytest.mark.parametrize("cov,height,exp", [(0.5, 0.5, 1.0), (0.5, 0.7, 1.0), (0.5, 0.75, 0.9), (0.5, 0.8, 0.8),
                                             (0.3, 0.85, 0.8), (0.0, 0.9, 0.8)])
 def test_height_bias_coverage(orc, cov, height, exp):

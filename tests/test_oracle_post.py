@@ -113,3 +113,23 @@ def test_radial_blur_taps_are_symmetric_about_the_pixel(mm, orc):
     acc = hdr[..., 0] / 1.1  # rgb = 0 here, so hdr.r = 1.1 * mean(alpha taps)
     inner = slice(100, 210)  # the widest tap (0.06 * dist * W px) stays inside the image
     assert np.allclose(acc[:, inner], alpha[:, inner], atol=2e-3)
+
+
+def test_series_pow_against_float64(orc):
+    # pow as the tonemap evaluates it (exp2(y log2 x) in float series) against float64 pow
+    xs = np.concatenate([np.geomspace(1e-30, 1e4, 4000), [1.0, 0.5, 2.0, 1e-40]]).astype(np.float32)
+    for y in (1 / 2.2, 2.2, 0.5, 3.0):
+        got = np.array([orc.L.orc_post_pow(float(x), y) for x in xs], np.float64)
+        want = np.power(xs.astype(np.float64), np.float32(y).astype(np.float64))
+        ok = want > 1e-37  # normal results
+        # float rounding of t = y log2 x moves 2^t by ln2 |t| 2^-24 relative: the bound grows with |t|
+        t = np.abs(y * np.log2(xs.astype(np.float64)))
+        bound = (1e-6 + 2.0 * t * np.log(2.0) * 2.0 ** -24) * want
+        assert np.all(np.abs(got[ok] - want[ok]) <= bound[ok]), y
+        if y == 1 / 2.2:  # the tonemap's gamma on the values it sees
+            tone = ok & (xs >= 1e-4) & (xs <= 4.0)
+            assert np.all(np.abs(got[tone] - want[tone]) <= 1e-6 * want[tone])
+    assert orc.L.orc_post_pow(0.0, 0.45) == 0.0
+    assert np.isnan(orc.L.orc_post_pow(-1.0, 0.45))
+    assert orc.L.orc_post_pow(float("inf"), 0.45) == float("inf")
+    assert abs(orc.L.orc_post_pow(1.0, 0.45) - 1.0) <= 6e-8
