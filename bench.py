@@ -157,6 +157,27 @@ def post_leg(mm, cam, sun, out, W, H, steps, warmup):
             "pass_gbs": round(moved / (ms * 1e-3) / 1e9, 1)}
 
 
+def amortized_leg(mm, ns, cam, sun, sky, prm, W, H, warmup):
+    """The reference's per-frame compute cost model (VulkanApplication.cpp:1046-1071): the previous
+    frame reprojected at full resolution, then one 1/16 pixel phase marched (mm_render_frame), over
+    16 consecutive frames (every phase once), textures resident. Reports ms per frame."""
+    import torch
+
+    frames = mm.CloudFrames(W, H, ns, params=prm)
+    sun = type(sun).from_buffer_copy(sun)  # the frames advance the phase in sun.color.a
+    for _ in range(max(warmup, 16)):
+        frames.frame(cam, sun, sky)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(32):
+        frames.frame(cam, sun, sky)
+    e1.record()
+    torch.cuda.synchronize()
+    return {"ms_per_frame": round(e0.elapsed_time(e1) / 32, 4), "frames": 32,
+            "what": "reprojection of the previous frame + 1/16 of the pixels marched (phase cycles 0..15)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +190,7 @@ def main():
     ap.add_argument("--tile-width", type=int, default=0, help="8, 16 or 32 (0 = library default)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work (0 = skip)")
     ap.add_argument("--no-post", action="store_true", help="skip the post-chain leg")
+    ap.add_argument("--no-amortized", action="store_true", help="skip the 1/16-phase frame leg")
     ap.add_argument("--surrogate-steps", type=int, default=4000,
                     help="Adam steps of the learned light-cone surrogate leg (0 = skip the leg)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -275,7 +297,10 @@ def main():
         "cpu_baseline": None,
         "surrogate": None,
         "post": None,
+        "amortized_frame": None,
     }
+    if not args.no_amortized:
+        rec["amortized_frame"] = amortized_leg(mm, ns, cam, sun, sky, prm, W, H, args.warmup)
     if not args.no_post:
         rec["post"] = post_leg(mm, cam, sun, out, W, H, args.steps, args.warmup)
     if args.surrogate_steps > 0 and not args.no_batch:
