@@ -628,7 +628,8 @@ __device__ __forceinline__ float cone_dal(const MarchConsts& c, const MarchTextu
     V3 center = ld3(c.camPos);
     center.y = c.earthCenter[1];
     float dal = 0.0f;
-#pragma unroll 1
+    // two independent samples in flight per lane (unrolling all six is slower: code size)
+#pragma unroll 2
     for (int i = 0; i < 6; i++) {
         V3 lsPos = currentPos + ((3.0f * stepSize) * ld3(c.coneOffsets[i]));
         V3 lsProj = proj_shell(c, lsPos, center);

@@ -235,15 +235,13 @@ struct HashCells {
 };
 
 struct LdsCells {
-    const float* a;
-    const float* b;
-    const float* c;
+    const float4* cell;      // (x, y, z, unused) per staged cell: one 16-byte LDS read per lookup
     int bx, by, bz, nx, ny;  // first staged cell and the staged extent in x and y
     __device__ __forceinline__ void point(int x, int y, int z, float& ux, float& uy, float& uz) const {
-        const int i = ((z - bz) * ny + (y - by)) * nx + (x - bx);
-        ux = a[i];
-        uy = b[i];
-        uz = c[i];
+        const float4 v = cell[((z - bz) * ny + (y - by)) * nx + (x - bx)];
+        ux = v.x;
+        uy = v.y;
+        uz = v.z;
     }
 };
 
@@ -424,13 +422,11 @@ struct StageMeta {
 };
 
 struct LowResLds {
-    const float* a;
-    const float* b;
-    const float* c;
+    const float4* cells;
     const StageMeta* meta;
     __device__ __forceinline__ LdsCells level(int l) const {
         const StageMeta& m = meta[l];
-        return LdsCells{a + m.off, b + m.off, c + m.off, m.bx, m.by, m.bz, m.nx, m.ny};
+        return LdsCells{cells + m.off, m.bx, m.by, m.bz, m.nx, m.ny};
     }
     __device__ __forceinline__ LdsCells perlin(int o) const { return level(o); }
     __device__ __forceinline__ LdsCells worley(int L, int k) const { return level(4 + 3 * L + k); }
@@ -485,7 +481,7 @@ __device__ __forceinline__ void store_low_res(const GenArgs& a, long i, const fl
 // place. Texels whose coordinate is below the apron width are also written to the apron copy
 // (x + dim, ...).
 constexpr int kBrickX = 8, kBrickY = 8, kBrickZ = 4;
-constexpr int kStageCap = 1536;  // staged cells per block: 18 KB of LDS
+constexpr int kStageCap = 1536;  // staged cells per block: 24 KB of LDS
 
 // Cells [floor(q(t0)) - pad, floor(q(t1)) + 1] of one axis, q(t) = ((2t + 1) / (2 dim)) * cells,
 // evaluated as the texel evaluates it (q is monotone in t, so every texel's cells lie inside).
@@ -497,7 +493,7 @@ __device__ __forceinline__ void axis_span(int t0, int t1, float inv, int cells, 
 }
 
 __global__ __launch_bounds__(256) void low_res_brick_kernel(GenArgs a) {
-    __shared__ float sa[kStageCap], sb[kStageCap], sc[kStageCap];
+    __shared__ float4 cells[kStageCap];
     __shared__ StageMeta meta[kLowLevels];
     __shared__ int total;
     const int tid = threadIdx.x;
@@ -539,9 +535,7 @@ __global__ __launch_bounds__(256) void low_res_brick_kernel(GenArgs a) {
                 HashGrads{low_level_cells(l), low_level_seed(a.seed, l)}.point(m.bx + jx, m.by + jy, m.bz + jz, u0, u1, u2);
             else
                 HashCells{low_level_cells(l), low_level_seed(a.seed, l)}.point(m.bx + jx, m.by + jy, m.bz + jz, u0, u1, u2);
-            sa[e] = u0;
-            sb[e] = u1;
-            sc[e] = u2;
+            cells[e] = make_float4(u0, u1, u2, 0.0f);
         }
     }
     __syncthreads();
@@ -550,7 +544,7 @@ __global__ __launch_bounds__(256) void low_res_brick_kernel(GenArgs a) {
     const float px = (float)(2 * x + 1) * inv, py = (float)(2 * y + 1) * inv, pz = (float)(2 * z + 1) * inv;
     float v[4];
     if (staged)
-        low_res_values(px, py, pz, LowResLds{sa, sb, sc, meta}, v);
+        low_res_values(px, py, pz, LowResLds{cells, meta}, v);
     else
         low_res_values(px, py, pz, LowResHash{a.seed}, v);
     const long sd = a.dim + a.apron;
