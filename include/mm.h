@@ -322,7 +322,8 @@ int mm_post_process(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const fl
  * at byte offset i * width * height * 4; upload it into an RGBA8 LINEAR texture and convert it to
  * another format set with mm_pack_texture. TGA: types 1/2/3/9/10/11, 8/15/16/24/32-bit pixels.
  * PNG: all colour types, bit depths 1-16 (16-bit samples keep their high byte, as stb's 8-bit
- * interface does), tRNS; interlaced PNGs are rejected (MM_E_FORMAT). */
+ * interface does), tRNS; interlaced PNGs are rejected (MM_E_FORMAT), images above 2^28 pixels
+ * with MM_E_DIM, and a failed allocation returns MM_E_SIZE. */
 int mm_image_info(const char* path, int32_t* width, int32_t* height);
 int mm_load_image_rgba8(const char* path, uint8_t* rgba, size_t bytes, int32_t* width, int32_t* height);
 int mm_load_volume_rgba8(const char* prefix, int32_t depth, uint8_t* rgba, size_t bytes, int32_t* width,

@@ -692,7 +692,11 @@ struct mm_view_plan {
 
 static int fill_views(const mm_view* views, int32_t n, const mm_noise_set* shared, const mm_march_params* params,
                       std::vector<mm::ViewDesc>& out, mm::MarchTextures& tx, int& fset, int& total) {
-    out.resize((size_t)n);
+    try {  // no exception crosses the C boundary
+        out.resize((size_t)n);
+    } catch (...) {
+        return MM_E_SIZE;
+    }
     total = 0;
     fset = -1;
     for (int i = 0; i < n; i++) {

@@ -22,6 +22,9 @@
 
 namespace {
 
+// Images above this many pixels are rejected before any allocation (16384 x 16384).
+constexpr size_t kMaxPixels = (size_t)1 << 28;
+
 struct Image {
     int w = 0, h = 0;
     std::vector<uint8_t> rgba;
@@ -48,6 +51,7 @@ int decode_tga(const std::vector<uint8_t>& b, Image& im) {
     const bool rle = type >= 9;
     const int base = rle ? type - 8 : type;
     if (base < 1 || base > 3 || w <= 0 || h <= 0) return MM_E_FORMAT;
+    if ((size_t)w * h > kMaxPixels) return MM_E_DIM;
     if (base == 1 && (cmapType != 1 || bits != 8)) return MM_E_FORMAT;
     if (base == 2 && bits != 15 && bits != 16 && bits != 24 && bits != 32) return MM_E_FORMAT;
     if (base == 3 && bits != 8 && bits != 16) return MM_E_FORMAT;
@@ -171,6 +175,7 @@ int decode_png(const std::vector<uint8_t>& b, Image& im) {
         pos += 12 + (size_t)n;
     }
     if (!seenIhdr || w <= 0 || h <= 0 || w > (1 << 16) || h > (1 << 16)) return MM_E_FORMAT;
+    if ((size_t)w * h > kMaxPixels) return MM_E_DIM;
     if (interlace != 0) return MM_E_FORMAT;
     int channels;
     switch (ctype) {
@@ -287,10 +292,15 @@ int decode_png(const std::vector<uint8_t>& b, Image& im) {
 }
 
 int decode(const char* path, Image& im) {
-    std::vector<uint8_t> b;
-    if (!read_file(path, b)) return MM_E_IO;
-    if (b.size() >= 8 && b[0] == 137 && b[1] == 'P' && b[2] == 'N' && b[3] == 'G') return decode_png(b, im);
-    return decode_tga(b, im);
+    // no exception crosses the C boundary: allocation failures become an error code
+    try {
+        std::vector<uint8_t> b;
+        if (!read_file(path, b)) return MM_E_IO;
+        if (b.size() >= 8 && b[0] == 137 && b[1] == 'P' && b[2] == 'N' && b[3] == 'G') return decode_png(b, im);
+        return decode_tga(b, im);
+    } catch (...) {
+        return MM_E_SIZE;
+    }
 }
 
 }  // namespace
@@ -324,19 +334,25 @@ int mm_load_volume_rgba8(const char* prefix, int32_t depth, uint8_t* rgba, size_
     if (!prefix || !rgba || !width || !height) return MM_E_NULL;
     if (depth <= 0) return MM_E_DIM;
     int w = 0, h = 0;
-    for (int i = 0; i < depth; i++) {
-        const std::string path = std::string(prefix) + "(" + std::to_string(i) + ").tga";
-        Image im;
-        const int e = decode(path.c_str(), im);
-        if (e) return e;
-        if (i == 0) {
-            w = im.w;
-            h = im.h;
-            if (bytes < (size_t)w * h * 4 * depth) return MM_E_SIZE;
-        } else if (im.w != w || im.h != h) {
-            return MM_E_DIM;
+    try {
+        for (int i = 0; i < depth; i++) {
+            char suffix[32];
+            snprintf(suffix, sizeof(suffix), "(%d).tga", i);
+            const std::string path = std::string(prefix) + suffix;
+            Image im;
+            const int e = decode(path.c_str(), im);
+            if (e) return e;
+            if (i == 0) {
+                w = im.w;
+                h = im.h;
+                if (bytes < (size_t)w * h * 4 * depth) return MM_E_SIZE;
+            } else if (im.w != w || im.h != h) {
+                return MM_E_DIM;
+            }
+            memcpy(rgba + (size_t)i * w * h * 4, im.rgba.data(), im.rgba.size());
         }
-        memcpy(rgba + (size_t)i * w * h * 4, im.rgba.data(), im.rgba.size());
+    } catch (...) {
+        return MM_E_SIZE;
     }
     *width = w;
     *height = h;

@@ -169,6 +169,18 @@ def test_tga_truncated_and_missing(mm, tmp_path):
     assert e.value.code == mm.MM_E_IO
 
 
+def test_oversized_header_is_rejected_before_allocating(mm, tmp_path):
+    hdr = bytearray(18)
+    hdr[2] = 2
+    hdr[12:16] = struct.pack("<HH", 65535, 65535)  # 4.3 G pixels claimed, no data
+    hdr[16] = 32
+    p = tmp_path / "huge.tga"
+    p.write_bytes(bytes(hdr))
+    with pytest.raises(mm.MMError) as e:
+        mm.load_image(str(p))
+    assert e.value.code == mm.MM_E_DIM
+
+
 def test_volume_slices(mm, tmp_path):
     rng = np.random.default_rng(9)
     vol = rng.integers(0, 256, (5, 6, 7, 4), dtype=np.uint8)
