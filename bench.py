@@ -157,6 +157,14 @@ def post_leg(mm, cam, sun, out, W, H, steps, warmup):
             "pass_gbs": round(moved / (ms * 1e-3) / 1e9, 1)}
 
 
+def march_kernel_name(args):
+    """The kernel a full-frame plain march launches: lock-step, cone-batched (1 lane per ray) or
+    ray-parallel (2 lanes per ray by default)."""
+    if args.no_batch:
+        return "march_kernel"
+    return "march_batched_kernel" if args.lanes_per_ray == 1 else "march_rp_kernel"
+
+
 def amortized_leg(mm, ns, cam, sun, sky, prm, W, H, warmup):
     """The reference's per-frame compute cost model (VulkanApplication.cpp:1046-1071): the previous
     frame reprojected at full resolution, then one 1/16 pixel phase marched (mm_render_frame), over
@@ -196,6 +204,7 @@ def main():
     ap.add_argument("--fset", default="fast", choices=["fast", "ref", "prec"])
     ap.add_argument("--no-batch", action="store_true", help="evaluate light cones in place (lock-step kernel)")
     ap.add_argument("--tile-width", type=int, default=0, help="8, 16 or 32 (0 = library default)")
+    ap.add_argument("--lanes-per-ray", type=int, default=0, help="1, 2, 4 or 8 (0 = library default)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work (0 = skip)")
     ap.add_argument("--no-post", action="store_true", help="skip the post-chain leg")
     ap.add_argument("--no-amortized", action="store_true", help="skip the 1/16-phase frame leg")
@@ -231,6 +240,7 @@ def main():
     if args.no_batch:
         prm.flags |= mm.MARCH_NO_BATCH
     prm.tile_width = args.tile_width
+    prm.lanes_per_ray = args.lanes_per_ray
     ns = mm.NoiseSet(args.fset)
     nparams = mm.noise_params(seed=1, weather_seed=1, coverage=0.5)
     out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
@@ -271,7 +281,7 @@ def main():
     achieved = alg_bytes / (march_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(args.fset, W, H),
-                "kernel": "march_kernel" if args.no_batch else "march_batched_kernel", "kernel_ms": round(march_ms, 4),
+                "kernel": march_kernel_name(args), "kernel_ms": round(march_ms, 4),
                 "algorithmic_bytes_per_launch": alg_bytes,
                 "bytes_per_density_sample": b["density"], "bytes_per_detail_sample": b["detail"]}
     rec = {
@@ -291,8 +301,8 @@ def main():
                                f"(Earth radius 6.371e6 m, shell 1500-4000 m, camera 200 m pitched +20 deg, 60 deg "
                                f"FOV, sun 25 deg, coverage 0.5, t = 0, weather seed 1)",
                    "width": W, "height": H, "format_set": args.fset,
-                   "kernel": "lock-step" if args.no_batch else "cone-batched", "coarse_steps_max": 128, "fine_steps_max": 256,
-                   "tile_width": args.tile_width or 8},
+                   "kernel": march_kernel_name(args), "coarse_steps_max": 128, "fine_steps_max": 256,
+                   "tile_width": args.tile_width or 8, "lanes_per_ray": 1 if args.no_batch else (args.lanes_per_ray or 2)},
         "ms_per_frame": round(ms_step, 4),
         "frames_per_s": round(1000.0 / ms_step, 2),
         "march_ms": round(march_ms, 4),

@@ -193,7 +193,11 @@ typedef struct mm_march_params {
                                    REF and FAST texture sets only */
     int32_t tile_width;         /* pixels per wave tile row: 8 (8x8, default when 0), 16 (16x4), 32 (32x2);
                                    the image does not depend on it */
-    int32_t _reserved[7];
+    int32_t lanes_per_ray;      /* 1, 2, 4 or 8 lanes march one ray, sampling the next trips ahead
+                                   and committing them in order (0 = default: 4 for one phase, else
+                                   2); plain cones only (other modes use 1); the image does not
+                                   depend on it */
+    int32_t _reserved[6];
 } mm_march_params;
 
 /* Preset with the reference's constants (compute-clouds.comp, bit-for-bit). */

@@ -91,7 +91,8 @@ class MarchParams(ctypes.Structure):
                 ("wind_strength", ctypes.c_float), ("weather_scale", ctypes.c_float), ("low_scale", ctypes.c_float),
                 ("curl_scale", ctypes.c_float), ("hi_scale", ctypes.c_float), ("subtexel_bits", ctypes.c_int32),
                 ("cone_mode", ctypes.c_int32), ("surrogate_threshold", ctypes.c_float),
-                ("surrogate", ctypes.c_void_p), ("tile_width", ctypes.c_int32), ("_reserved", ctypes.c_int32 * 7)]
+                ("surrogate", ctypes.c_void_p), ("tile_width", ctypes.c_int32), ("lanes_per_ray", ctypes.c_int32),
+                ("_reserved", ctypes.c_int32 * 6)]
 
 
 class Scene(ctypes.Structure):

@@ -701,34 +701,58 @@ __device__ __forceinline__ void march_init(const MarchConsts& c, const RaySetup&
     m.active = r.march && (m.t < r.tEnd);
 }
 
-// One trip of the reference's for-loop, including its increment. A fine-mode hit that survives the
-// hi-res erosion returns a light-cone job (position and step size at this trip, the in-scatter
-// factor and the mix weight 1 - accum) instead of evaluating the cone in place; its result only
-// enters the transmittance recurrence, never the loop's control flow.
+// The samples of one loop trip at t (comp:406-431): position, projected point and relative height,
+// the wind-offset density and, in fine mode where it is positive, the hi-res eroded density. It
+// depends only on t, the mode and the step size, so several trips' samples can be taken at once.
+struct TripSample {
+    V3 pos;
+    float rHeight, lo, dens;
+    int nd, nh;  // density / detail samples taken (counted once the trip commits)
+};
+
 template <int FSET, bool COUNT>
-__device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchTextures& tx, const RaySetup& r,
-                                           MarchState& m, V3& jobPos, float& jobStep, float& jobX0, float& jobW,
-                                           float& jobRh, float& jobLo, float& jobDens, int& nd, int& nh) {
-    bool job = false;
-    // hard bound on loop trips: guarantees termination even if t += stepSize stops advancing t
-    if (++m.iterations > MM_MAX_MARCH_ITERATIONS) { m.active = false; return false; }
-    const bool coarse = m.noHits;
-    const bool bounded = (c.flags & MM_MARCH_BOUNDED) != 0;
-    if (bounded) {
-        if (coarse) { if (c.maxCoarse >= 0 && ++m.mc0 > c.maxCoarse) { m.active = false; return false; } }
-        else { if (c.maxFine >= 0 && ++m.mc1 > c.maxFine) { m.active = false; return false; } }
-    }
+__device__ __forceinline__ TripSample trip_eval(const MarchConsts& c, const MarchTextures& tx, const RaySetup& r,
+                                                bool noHits, float stepSize, float t) {
+    TripSample s;
+    s.nd = s.nh = 0;
     const V3 camPos = ld3(c.camPos);
     V3 center = camPos;
     center.y = c.earthCenter[1];
-    V3 currentPos = camPos + (m.t * r.rd);
-    V3 currentProj = proj_shell(c, currentPos, center);
-    float rHeight = rel_height(c, currentPos, currentProj);
-    V3 wo = wind_offset(c, rHeight);
-    float density = cloud_test<FSET, COUNT>(c, tx, currentPos + wo, rHeight, center, nd);
-    float loDensity = density;
+    s.pos = camPos + (t * r.rd);
+    const V3 proj = proj_shell(c, s.pos, center);
+    s.rHeight = rel_height(c, s.pos, proj);
+    const V3 wo = wind_offset(c, s.rHeight);
+    s.lo = cloud_test<FSET, COUNT>(c, tx, s.pos + wo, s.rHeight, center, s.nd);
+    s.dens = s.lo;
+    if (s.lo > 0.0f && !noHits) s.dens = cloud_hires<FSET, COUNT>(c, tx, s.pos + wo, stepSize, s.lo, s.rHeight, s.nh);
+    return s;
+}
+
+// The rest of the trip (comp:386-482) given its samples, including the loop increment. A fine-mode
+// hit that survives the hi-res erosion returns a light-cone job (the in-scatter factor and the mix
+// weight 1 - accum at this trip) instead of evaluating the cone in place; its result only enters the
+// transmittance recurrence, never the loop's control flow.
+// The checks that end a trip before it samples: the hard bound on loop trips (termination even if
+// t += stepSize stops advancing t) and, in bounded mode, the coarse / fine step budgets.
+__device__ __forceinline__ bool trip_begin(const MarchConsts& c, MarchState& m) {
+    if (++m.iterations > MM_MAX_MARCH_ITERATIONS) { m.active = false; return false; }
+    if (c.flags & MM_MARCH_BOUNDED) {
+        if (m.noHits) { if (c.maxCoarse >= 0 && ++m.mc0 > c.maxCoarse) { m.active = false; return false; } }
+        else { if (c.maxFine >= 0 && ++m.mc1 > c.maxFine) { m.active = false; return false; } }
+    }
+    return true;
+}
+
+// After trip_begin: the state update of the trip with its samples (low-res density lo, eroded
+// density dens). A job's mix weight 1 - accum is returned in jobW; the caller takes its in-scatter
+// factor from the trip's own samples.
+__device__ __forceinline__ bool trip_commit(const MarchConsts& c, const RaySetup& r, MarchState& m, float lo,
+                                            float dens, float& jobW) {
+    bool job = false;
+    const bool coarse = m.noHits;
+    const bool bounded = (c.flags & MM_MARCH_BOUNDED) != 0;
     bool cont = false;  // `continue`: skip the loop tail but still run the increment
-    if (density > 0.0f) {
+    if (lo > 0.0f) {
         m.misses = 0;
         if (m.noHits) {
             m.t -= m.stepSize;
@@ -736,19 +760,12 @@ __device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchText
             m.noHits = false;
             cont = true;
         } else {
-            density = cloud_hires<FSET, COUNT>(c, tx, currentPos + wo, m.stepSize, density, rHeight, nh);
-            if (density < 0.0001f) {
+            if (dens < 0.0001f) {
                 cont = true;
             } else {
                 job = true;
-                jobPos = currentPos;
-                jobStep = m.stepSize;
-                jobX0 = inscatter_hg(loDensity, rHeight, r.HG);
                 jobW = 1.0f - m.accum;
-                jobRh = rHeight;
-                jobLo = loDensity;
-                jobDens = density;
-                m.accum += density;
+                m.accum += dens;
             }
         }
     } else if (!m.noHits) {
@@ -768,6 +785,28 @@ __device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchText
     }
     m.t += m.stepSize;
     if (!(m.t < r.tEnd)) m.active = false;
+    return job;
+}
+
+// One trip of the reference's for-loop: its samples, then its state update.
+template <int FSET, bool COUNT>
+__device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchTextures& tx, const RaySetup& r,
+                                           MarchState& m, V3& jobPos, float& jobStep, float& jobX0, float& jobW,
+                                           float& jobRh, float& jobLo, float& jobDens, int& nd, int& nh) {
+    if (!trip_begin(c, m)) return false;
+    const TripSample s = trip_eval<FSET, COUNT>(c, tx, r, m.noHits, m.stepSize, m.t);
+    const float step = m.stepSize;
+    const bool job = trip_commit(c, r, m, s.lo, s.dens, jobW);
+    if (job) {
+        jobX0 = inscatter_hg(s.lo, s.rHeight, r.HG);
+        jobPos = s.pos;
+        jobStep = step;
+        jobRh = s.rHeight;
+        jobLo = s.lo;
+        jobDens = s.dens;
+    }
+    nd += s.nd;
+    nh += s.nh;
     return job;
 }
 
@@ -1024,6 +1063,111 @@ __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, 
     march_tile<FSET, COUNT, MODE>(c, tx, tile, out, counts);
 }
 
+// ---------------------------------------------------------------- ray-parallel march
+// LPR lanes per ray (2 or 4), 64 / LPR rays per wave in an 8-wide tile. Every lane of a ray holds
+// the ray's whole state. Each loop round, lane k of the ray samples trip k ahead: at t plus k times
+// the step (the same float additions the loop makes), in the current mode. The ray's lanes then
+// commit the trips in order, identically, from the exchanged samples, and stop at the first trip
+// whose mode, step or t no longer match what was sampled (a mode switch or the end of the ray): the
+// committed trips are exactly the loop's, so the image is the one-lane-per-ray image. It shortens
+// a ray's critical path by up to LPR, which is what bounds a single 1/16 phase (two waves per SIMD,
+// each as long as its longest ray). Light-cone jobs go to the wave's queue like the batched kernel's.
+template <int FSET, bool COUNT, int LPR>
+__device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchTextures& tx, const int tile,
+                                              float* __restrict__ out, unsigned long long* __restrict__ counts) {
+    static_assert(LPR == 2 || LPR == 4 || LPR == 8, "lanes per ray");
+    constexpr int TH = 64 / LPR / 8;  // tile: 8 x TH rays
+    __shared__ ConeQueuePlain q;
+    const int lane = threadIdx.x, ray = lane / LPR, k = lane % LPR, first = lane - k;
+    const int gi = (tile % c.tilesX) * 8 + (ray & 7);
+    const int gj = (tile / c.tilesX) * TH + (ray >> 3);
+    const bool valid = gi < c.gridW && gj < c.gridH;
+    int px = 0, py = 0;
+    if (valid) tile_pixel(c, gi, gj, px, py);
+    int nd = 0, nh = 0, nj = 0, nc = 0;
+    RaySetup r;
+    MarchState m;
+    m.active = false;
+    if (valid) {
+        r = ray_setup(c, tx, px, py);
+        if (!r.march && k == 0)
+            reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = make_float4(r.bg.x, r.bg.y, r.bg.z, r.alpha);
+        march_init(c, r, m);
+    }
+    float transmittance = 1.0f;
+    int qn = 0;  // queued jobs (wave-uniform)
+    for (;;) {
+        while (qn < 64 && __any(m.active)) {
+            const bool noHits0 = m.noHits;
+            const float step0 = m.stepSize;
+            float tk = m.t;
+            for (int j = 0; j < k; j++) tk += step0;
+            TripSample s;
+            s.pos = mk(0.f, 0.f, 0.f);
+            s.rHeight = s.lo = s.dens = 0.0f;
+            s.nd = s.nh = 0;
+            if (m.active) s = trip_eval<FSET, COUNT>(c, tx, r, noHits0, step0, tk);
+            float lo[LPR], dens[LPR];
+#pragma unroll
+            for (int j = 0; j < LPR; j++) {
+                lo[j] = __shfl(s.lo, first + j);
+                dens[j] = __shfl(s.dens, first + j);
+            }
+            bool mine = false;  // this lane's trip committed with a light-cone job
+            float jw = 0.0f;
+            float tj = m.t;
+#pragma unroll
+            for (int j = 0; j < LPR; j++) {
+                if (!m.active) break;
+                if (j > 0 && !(m.noHits == noHits0 && m.stepSize == step0 && m.t == tj)) break;
+                if (!trip_begin(c, m)) break;
+                float w = 0.0f;
+                const bool job = trip_commit(c, r, m, lo[j], dens[j], w);
+                if (j == k) {
+                    if (COUNT) { nd += s.nd; nh += s.nh; }
+                    mine = job;
+                    jw = w;
+                }
+                tj = tj + step0;
+            }
+            const unsigned long long mask = __ballot(mine);
+            if (mine) {
+                const int slot = qn + __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                q.x[slot] = s.pos.x; q.y[slot] = s.pos.y; q.z[slot] = s.pos.z; q.step[slot] = step0;
+                q.x0[slot] = inscatter_hg(s.lo, s.rHeight, r.HG); q.w[slot] = jw; q.owner[slot] = lane;
+            }
+            qn += __popcll(mask);
+        }
+        if (qn == 0) break;
+        __syncthreads();
+        const int nb = qn < 64 ? qn : 64;
+        run_batch<FSET, COUNT, CONE_PLAIN>(c, tx, q, nb, valid ? r.beerBlend : 0.0f, nd, nh, nj, nc);
+        for (int j = 0; j < nb; j++)
+            if (q.owner[j] / LPR == ray) transmittance = mixf(transmittance, q.res[j], q.w[j]);
+        queue_shift(q, nb, qn - nb);
+        qn -= nb;
+    }
+    if (valid && r.march && k == 0)
+        reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
+    if (COUNT) {
+        atomicAdd(&counts[0], (unsigned long long)nd);
+        atomicAdd(&counts[1], (unsigned long long)nh);
+        if (lane == 0) {
+            atomicAdd(&counts[2], (unsigned long long)nj);
+            atomicAdd(&counts[3], (unsigned long long)nc);
+        }
+    }
+}
+
+template <int FSET, bool COUNT, int LPR>
+__global__ __launch_bounds__(64) void march_rp_kernel(const MarchConsts c, const MarchTextures tx,
+                                                      float* __restrict__ out, unsigned long long* __restrict__ counts) {
+    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
+    if (tile >= c.tilesX * c.tilesY) return;  // uniform across the block
+    march_rp_tile<FSET, COUNT, LPR>(c, tx, tile, out, counts);
+}
+
 #ifndef MM_SURROGATE_WAVES_PER_EU
 #define MM_SURROGATE_WAVES_PER_EU 4
 #endif
@@ -1169,6 +1313,12 @@ int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned 
     dim3 grid((unsigned)(8 * c.tilesPerXcd)), block(64);
     if (c.coneMode == CONE_PLAIN && (c.flags & MM_MARCH_NO_BATCH))
         hipLaunchKernelGGL((march_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
+    else if (c.coneMode == CONE_PLAIN && c.lanesPerRay == 8)
+        hipLaunchKernelGGL((march_rp_kernel<FSET, COUNT, 8>), grid, block, 0, s, c, t, out, counts);
+    else if (c.coneMode == CONE_PLAIN && c.lanesPerRay == 4)
+        hipLaunchKernelGGL((march_rp_kernel<FSET, COUNT, 4>), grid, block, 0, s, c, t, out, counts);
+    else if (c.coneMode == CONE_PLAIN && c.lanesPerRay == 2)
+        hipLaunchKernelGGL((march_rp_kernel<FSET, COUNT, 2>), grid, block, 0, s, c, t, out, counts);
     else if (c.coneMode == CONE_PLAIN)
         hipLaunchKernelGGL((march_batched_kernel<FSET, COUNT, CONE_PLAIN>), grid, block, 0, s, c, t, out, counts);
     else if constexpr (FSET != mm::FSET_PREC) {

@@ -36,6 +36,7 @@ struct MarchConsts {
     int phase;                   // -1 = all pixels
     int gridW, gridH;            // marched pixel grid
     int tileShift;                // tiles of (1 << tileShift) x (64 >> tileShift) pixels: 3, 4 or 5
+    int lanesPerRay;              // 1, or 2 / 4 / 8 for the ray-parallel kernel (tiles of 8 x (8 / lanesPerRay) rays)
     int tilesX, tilesY;           // tiles of the marched grid
     int tilesPerXcd;              // ceil(tilesX * tilesY / 8)
     float probeOffsets[3][3];     // basis * (0, k, 0), k = 0.6, 2.5, 6: surrogate density probes

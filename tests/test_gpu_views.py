@@ -107,24 +107,28 @@ def test_views_surrogate_threshold_zero(mm, vs):
     plan.close()
 
 
-def test_tile_shape_does_not_change_the_image(mm, gpu):
+def test_tile_shape_and_lanes_per_ray_do_not_change_the_image(mm, gpu):
     import torch
 
     W, H = 200, 120
     ns = mm.NoiseSet("fast")
     ns.generate(mm.noise_params(seed=1, weather_seed=3, coverage=0.6), mm.current_stream())
     cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H))
-    outs, cnts = [], []
-    for tw in (8, 16, 32):
-        for flags in (0, mm.MARCH_NO_BATCH):
-            prm = mm.march_params("scenario", tile_width=tw)
+    variants = [(tw, flags, 1) for tw in (8, 16, 32) for flags in (0, mm.MARCH_NO_BATCH)]
+    variants += [(8, 0, 2), (8, 0, 4), (8, 0, 8), (8, mm.MARCH_BOUNDED, 1), (8, mm.MARCH_BOUNDED, 4)]
+    for phase in (mm.PHASE_ALL, 5):
+        outs, cnts = [], []
+        for tw, flags, lanes in variants:
+            prm = mm.march_params("scenario", tile_width=tw, lanes_per_ray=lanes)
             prm.flags |= flags
             o = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
             c = torch.zeros(4, dtype=torch.int64, device="cuda")
-            mm.march_clouds(cam, sun, sky, ns, o, W, H, params=prm, counts=c)
+            mm.march_clouds(cam, sun, sky, ns, o, W, H, phase=phase, params=prm, counts=c)
             outs.append(o)
             cnts.append(c)
-    torch.cuda.synchronize()
-    for o, c in zip(outs[1:], cnts[1:]):
-        assert torch.equal(o, outs[0])
-        assert torch.equal(c, cnts[0])  # same samples, same cone jobs
+        torch.cuda.synchronize()
+        ref_o, ref_c, bnd_o, bnd_c = outs[0], cnts[0], outs[-2], cnts[-2]
+        for (tw, flags, lanes), o, c in zip(variants, outs, cnts):
+            want_o, want_c = (bnd_o, bnd_c) if flags & mm.MARCH_BOUNDED else (ref_o, ref_c)
+            assert torch.equal(o, want_o), (phase, tw, flags, lanes)
+            assert torch.equal(c, want_c), (phase, tw, flags, lanes)  # same samples, same cone jobs
