@@ -409,7 +409,7 @@ static int build_march(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const
     c.tileShift = params->tile_width == 16 ? 4 : params->tile_width == 32 ? 5 : 3;
     c.tilesX = (c.gridW + (1 << c.tileShift) - 1) >> c.tileShift;
     c.tilesY = (c.gridH + (64 >> c.tileShift) - 1) / (64 >> c.tileShift);
-    c.tilesPerXcd = (c.tilesX * c.tilesY + 7) / 8;
+    c.tiles = c.tilesX * c.tilesY;
 
     memset(&t, 0, sizeof(t));
     t.weather = view_of(tex->weather);
@@ -438,7 +438,7 @@ static void ray_parallel_tiles(mm::MarchConsts& c, const mm_march_params* params
         c.tileShift = 3;
         c.tilesX = (c.gridW + 7) >> 3;
         c.tilesY = (c.gridH + th - 1) / th;
-        c.tilesPerXcd = (c.tilesX * c.tilesY + 7) / 8;
+        c.tiles = c.tilesX * c.tilesY;
     }
 }
 

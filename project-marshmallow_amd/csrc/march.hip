@@ -810,10 +810,10 @@ __device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchText
     return job;
 }
 
-// Tile order: the hardware deals workgroups round-robin to the 8 XCDs (block b -> XCD b mod 8), so
-// block b takes tile (b mod 8) * per + b / 8 and every XCD marches a contiguous band of tiles whose
-// texel footprint stays in its own L2.
-__device__ __forceinline__ int xcd_tile(int b, int per) { return (b & 7) * per + (b >> 3); }
+// Tile order: block b marches tile b (row-major). The hardware deals workgroups round-robin to the 8
+// XCDs, so consecutive tiles run on different XCDs and every XCD gets an even share of every image
+// region. (A contiguous band of tiles per XCD keeps each L2's footprint compact, but the band that
+// holds the long rays above the horizon then finishes last: 1.86x slower on the 1080p frame.)
 
 __device__ __forceinline__ void tile_pixel(const MarchConsts& c, int gi, int gj, int& px, int& py) {
     if (c.phase >= 0) {
@@ -830,8 +830,8 @@ __device__ __forceinline__ void tile_pixel(const MarchConsts& c, int gi, int gj,
 template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
                                                    unsigned long long* __restrict__ counts) {
-    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
-    if (tile >= c.tilesX * c.tilesY) return;
+    const int tile = blockIdx.x;
+    if (tile >= c.tiles) return;
     const int lane = threadIdx.x;
     const int gi = (tile % c.tilesX) * (1 << c.tileShift) + (lane & ((1 << c.tileShift) - 1));
     const int gj = (tile / c.tilesX) * (64 >> c.tileShift) + (lane >> c.tileShift);
@@ -1058,8 +1058,8 @@ template <int FSET, bool COUNT, int MODE>
 __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, const MarchTextures tx,
                                                            float* __restrict__ out,
                                                            unsigned long long* __restrict__ counts) {
-    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
-    if (tile >= c.tilesX * c.tilesY) return;  // uniform across the block
+    const int tile = blockIdx.x;
+    if (tile >= c.tiles) return;  // uniform across the block
     march_tile<FSET, COUNT, MODE>(c, tx, tile, out, counts);
 }
 
@@ -1163,8 +1163,8 @@ __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchT
 template <int FSET, bool COUNT, int LPR>
 __global__ __launch_bounds__(64) void march_rp_kernel(const MarchConsts c, const MarchTextures tx,
                                                       float* __restrict__ out, unsigned long long* __restrict__ counts) {
-    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
-    if (tile >= c.tilesX * c.tilesY) return;  // uniform across the block
+    const int tile = blockIdx.x;
+    if (tile >= c.tiles) return;  // uniform across the block
     march_rp_tile<FSET, COUNT, LPR>(c, tx, tile, out, counts);
 }
 
@@ -1177,8 +1177,8 @@ template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
 void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
                             unsigned long long* __restrict__ counts) {
-    const int tile = xcd_tile(blockIdx.x, c.tilesPerXcd);
-    if (tile >= c.tilesX * c.tilesY) return;
+    const int tile = blockIdx.x;
+    if (tile >= c.tiles) return;
     march_tile<FSET, COUNT, CONE_SURROGATE>(c, tx, tile, out, counts);
 }
 
@@ -1257,16 +1257,14 @@ __global__ __launch_bounds__(256) void cloud_shadow_kernel(const mm::ShadowConst
 }
 
 // ------------------------------------------------------------------ multi-view kernel
-// One launch over the tiles of all views: block b takes tile g = (b & 7) * per + b / 8 (each XCD
-// gets a contiguous range of tiles, so mostly one view's weather map per L2), finds its view by
-// binary search over the views' first-tile indices, and marches the tile with that view's
-// constants and weather map. The hardware dispatcher hands blocks to CUs as slots free up,
+// One launch over the tiles of all views: block b takes tile b of the concatenated tile lists, finds
+// its view by binary search over the views' first-tile indices, and marches the tile with that
+// view's constants and weather map. The hardware dispatcher hands blocks to CUs as slots free up,
 // which is the global work queue; no counter or persistent loop is needed.
 template <int FSET, bool COUNT, int MODE>
 __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, int nViews, const MarchTextures& tx0,
                                             int total, unsigned long long* __restrict__ counts) {
-    const int per = (total + 7) / 8;
-    const int g = xcd_tile(blockIdx.x, per);
+    const int g = blockIdx.x;
     if (g >= total) return;  // uniform across the block
     int lo = 0, hi = nViews - 1;  // last view whose first tile is <= g
     while (lo < hi) {
@@ -1310,7 +1308,7 @@ const void* views_kernel_ptr_c(int mode, bool count) {
 
 template <int FSET, bool COUNT>
 int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts, hipStream_t s) {
-    dim3 grid((unsigned)(8 * c.tilesPerXcd)), block(64);
+    dim3 grid((unsigned)c.tiles), block(64);
     if (c.coneMode == CONE_PLAIN && (c.flags & MM_MARCH_NO_BATCH))
         hipLaunchKernelGGL((march_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
     else if (c.coneMode == CONE_PLAIN && c.lanesPerRay == 8)
@@ -1356,7 +1354,7 @@ int launch_views(int fset, int mode, const ViewDesc* views, int nViews, const Ma
                     : fset == FSET_FAST ? views_kernel_ptr_c<FSET_FAST>(mode, counts != nullptr)
                                         : views_kernel_ptr_c<FSET_PREC>(mode, counts != nullptr);
     if (!k) return MM_E_FORMAT;
-    const unsigned grid = (unsigned)(8 * ((total + 7) / 8));
+    const unsigned grid = (unsigned)total;
     void* args[] = {(void*)&views, (void*)&nViews, (void*)&t, (void*)&total, (void*)&counts};
     if (hipLaunchKernel(k, dim3(grid), dim3(64), args, 0, s) != hipSuccess) return MM_E_DEVICE;
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
