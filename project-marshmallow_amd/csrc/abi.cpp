@@ -264,6 +264,31 @@ struct CollectArgs {
 constexpr int kDefaultFrameLanesPerRay = 2;
 constexpr int kDefaultPhaseLanesPerRay = 4;
 
+// Launch order of the tile rows: from the row just above the horizon up the sky (the longest rays
+// first), then the rows below the horizon, whose rays end at once. The sign of each tile row's ray
+// elevation is taken at the image's centre column (comp:305-322 with sx = 0). Any order gives the
+// same image; without a row above the horizon the order is top-down.
+static void horizon_first_rows(mm::MarchConsts& c) {
+    c.rowStart = 0;
+    c.rowDir = 1;
+    const int th = c.lanesPerRay > 1 ? 64 / c.lanesPerRay / 8 : (64 >> c.tileShift);
+    auto up = [&](int tr) {  // y of the unnormalised ray direction at the tile row's middle
+        int gj = tr * th + th / 2;
+        if (gj > c.gridH - 1) gj = c.gridH - 1;
+        const int py = c.phase >= 0 ? gj * 4 + (c.phase >> 2) : gj;
+        const double sy = 2.0 * py / c.height - 1.0;
+        return (double)c.refPoint[1] - sy * c.tanH * c.camUp[1] - c.camPos[1];
+    };
+    if (c.tilesY < 2) return;
+    const bool upright = up(0) >= up(c.tilesY - 1);  // elevation falls from the top row down
+    int start = -1;
+    for (int tr = 0; tr < c.tilesY; tr++)
+        if (up(tr) > 0.0 && (start < 0 || upright)) start = tr;  // upright: the lowest such row
+    if (start < 0) return;
+    c.rowStart = start;
+    c.rowDir = upright ? -1 : 1;
+}
+
 static int build_march(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
                        const mm_noise_set* tex, const mm_texture* night_sky, const float* out, int32_t width,
                        int32_t height, int32_t pitch, int32_t phase, const mm_march_params* params,
@@ -410,6 +435,7 @@ static int build_march(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const
     c.tilesX = (c.gridW + (1 << c.tileShift) - 1) >> c.tileShift;
     c.tilesY = (c.gridH + (64 >> c.tileShift) - 1) / (64 >> c.tileShift);
     c.tiles = c.tilesX * c.tilesY;
+    horizon_first_rows(c);
 
     memset(&t, 0, sizeof(t));
     t.weather = view_of(tex->weather);
@@ -454,6 +480,7 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
                               t, fset, empty);
     if (e || empty) return e;
     ray_parallel_tiles(c, params, phase);
+    horizon_first_rows(c);
     reset_error_state();
     return mm::launch_march(fset, c, t, out, counts, stream);
 }

@@ -38,7 +38,8 @@ struct MarchConsts {
     int tileShift;                // tiles of (1 << tileShift) x (64 >> tileShift) pixels: 3, 4 or 5
     int lanesPerRay;              // 1, or 2 / 4 / 8 for the ray-parallel kernel (tiles of 8 x (8 / lanesPerRay) rays)
     int tilesX, tilesY;           // tiles of the marched grid
-    int tiles;                    // tilesX * tilesY: one block per tile, row-major
+    int tiles;                    // tilesX * tilesY: one block per tile
+    int rowStart, rowDir;         // launch order of tile rows: logical row r is row (rowStart + rowDir r) mod tilesY
     float probeOffsets[3][3];     // basis * (0, k, 0), k = 0.6, 2.5, 6: surrogate density probes
     // light-cone mode: MM_CONE_PLAIN, MM_CONE_SURROGATE, or 2 = plain + record surrogate training samples
     int coneMode;
