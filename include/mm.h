@@ -194,7 +194,7 @@ typedef struct mm_march_params {
     int32_t tile_width;         /* pixels per wave tile row: 8 (8x8, default when 0), 16 (16x4), 32 (32x2);
                                    the image does not depend on it */
     int32_t lanes_per_ray;      /* 1, 2, 4 or 8 lanes march one ray, sampling the next trips ahead
-                                   and committing them in order (0 = default: 4 for one phase, else
+                                   and committing them in order (0 = default: 8 for one phase, else
                                    2); plain cones only (other modes use 1); the image does not
                                    depend on it */
     int32_t _reserved[6];
@@ -335,9 +335,9 @@ int mm_load_volume_rgba8(const char* prefix, int32_t depth, uint8_t* rgba, size_
 
 /* ---------------------------------------------------------------- many views in one launch
  * No counterpart in the reference (which dispatches one view per frame, VulkanApplication.cpp:
- * 1061-1071): many views of any resolution marched by one launch over the 64-pixel tiles of all
- * views, handed to the CUs by the hardware dispatcher as slots free up. Views share the curl and noise volumes and bring their own uniforms, weather map
- * and output image. Each view's image is identical to mm_march_clouds of that view with the same
+ * 1061-1071): many views of any resolution marched by one launch over the tiles of all views (one
+ * wave each), handed to the CUs by the hardware dispatcher as slots free up. Views share the curl
+ * and noise volumes and bring their own uniforms, weather map and output image. Each view's image is identical to mm_march_clouds of that view with the same
  * params. A plan holds the per-view constants in device memory; mm_march_views enqueues one kernel
  * (no allocation, no host transfer, no memset), so it can be captured into a HIP graph and
  * replayed. */
@@ -354,13 +354,14 @@ typedef struct mm_view {
 typedef struct mm_view_plan mm_view_plan;
 
 /* shared: curl, low_res and hi_res are used (its weather entry is ignored). params applies to every
- * view (NULL = reference preset); tile shape from params->tile_width. */
+ * view (NULL = reference preset); every view is marched with params->lanes_per_ray lanes per ray
+ * (0 = 1, whatever the views' phases), with one lane in 64-ray tiles params->tile_width wide. */
 int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_set* shared,
                         const mm_march_params* params, mm_view_plan** plan, void* stream);
 /* Replace the views' uniforms / outputs (same count) without reallocating. Synchronises the stream. */
 int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_views, const mm_march_params* params,
                         void* stream);
-/* Total 64-pixel tiles and the launch's grid (waves) of the plan. */
+/* Total tiles of the plan and the launch's grid (one wave per tile). */
 int mm_view_plan_info(const mm_view_plan* plan, int32_t* total_tiles, int32_t* grid_waves);
 /* counts: NULL or device uint64[4] accumulators as for mm_march_clouds_counted. */
 int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* stream);

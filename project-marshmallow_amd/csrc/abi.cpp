@@ -260,9 +260,12 @@ struct CollectArgs {
 // Validates one dispatch and derives its per-dispatch constants and texture views. `empty` is set
 // when the phase selects no pixel of the image (nothing to launch).
 // Default lanes per ray (see ray_parallel_tiles), measured on the 1080p scenario frame: every pixel
-// 1 / 2 / 4 lanes 3.39 / 3.30 / 4.02 ms; one 1/16 phase 0.96 / 0.68 / 0.53 (8 lanes: 0.56) ms.
+// 1 / 2 / 4 / 8 lanes 1.85 / 1.73 / 2.07 / 2.91 ms; one 1/16 phase 0.93 / 0.56 / 0.40 / 0.33 ms.
 constexpr int kDefaultFrameLanesPerRay = 2;
-constexpr int kDefaultPhaseLanesPerRay = 4;
+constexpr int kDefaultPhaseLanesPerRay = 8;
+// A views launch has many more waves (64 views of 256x256 to 1920x1080: 33.6 M pixels, 1 / 2 / 4
+// lanes 20.4 / 21.7 / 29.1 ms): latency is hidden already and one lane per ray wastes no samples.
+constexpr int kDefaultViewsLanesPerRay = 1;
 
 // Launch order of the tile rows: from the row just above the horizon up the sky (the longest rays
 // first), then the rows below the horizon, whose rays end at once. The sign of each tile row's ray
@@ -450,14 +453,20 @@ static int build_march(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const
     return MM_OK;
 }
 
-// Lanes per ray of a single-view plain march (params.lanes_per_ray; 0 = the default: 4 for a single
-// 1/16 phase, where there are few waves and each lasts as long as its longest ray, else 2) and the
-// ray-parallel kernel's tiles of 8 x (8 / lanes) rays. The image does not depend on it.
-static void ray_parallel_tiles(mm::MarchConsts& c, const mm_march_params* params, int32_t phase) {
+// Lanes per ray of a plain march (params.lanes_per_ray; 0 = the default: 8 for a single 1/16 phase,
+// where there are few waves and each lasts as long as its longest ray, else 2); 1 for the other cone
+// modes and the lock-step kernel. The image does not depend on it.
+static int lanes_for(const mm::MarchConsts& c, const mm_march_params* params, int32_t phase) {
     int lpr = params ? params->lanes_per_ray : 0;
     if (lpr == 0)
         lpr = (phase >= 0 || phase == MM_PHASE_FROM_UBO) ? kDefaultPhaseLanesPerRay : kDefaultFrameLanesPerRay;
     if (c.coneMode != MM_CONE_PLAIN || (c.flags & MM_MARCH_NO_BATCH)) lpr = 1;
+    return lpr;
+}
+
+// The ray-parallel kernel's tiles of 8 x (8 / lanes) rays (lanes 1 keeps the 64-ray tiles) and the
+// launch order of their rows.
+static void set_lanes(mm::MarchConsts& c, int lpr) {
     c.lanesPerRay = lpr;
     if (lpr > 1) {
         const int th = 64 / lpr / 8;
@@ -466,6 +475,7 @@ static void ray_parallel_tiles(mm::MarchConsts& c, const mm_march_params* params
         c.tilesY = (c.gridH + th - 1) / th;
         c.tiles = c.tilesX * c.tilesY;
     }
+    horizon_first_rows(c);
 }
 
 static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
@@ -479,8 +489,7 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
     const int e = build_march(camera, sun, sky, tex, night_sky, out, width, height, pitch, phase, params, collect, c,
                               t, fset, empty);
     if (e || empty) return e;
-    ray_parallel_tiles(c, params, phase);
-    horizon_first_rows(c);
+    set_lanes(c, lanes_for(c, params, phase));
     reset_error_state();
     return mm::launch_march(fset, c, t, out, counts, stream);
 }
@@ -738,15 +747,17 @@ int mm_post_process(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const fl
 }
 
 struct mm_view_plan {
-    int n_views = 0, total_tiles = 0, fset = 0, mode = 0;
+    int n_views = 0, total_tiles = 0, fset = 0, mode = 0, lanes = 1;
     mm::ViewDesc* d_views = nullptr;  // device: ViewDesc[n_views]
     mm::MarchTextures shared;
     mm_noise_set shared_set;          // the shared textures as given (curl, low_res, hi_res)
     int device = 0;
 };
 
+// Every view of a plan is marched with the same lanes per ray: params.lanes_per_ray, or
+// kDefaultViewsLanesPerRay.
 static int fill_views(const mm_view* views, int32_t n, const mm_noise_set* shared, const mm_march_params* params,
-                      std::vector<mm::ViewDesc>& out, mm::MarchTextures& tx, int& fset, int& total) {
+                      std::vector<mm::ViewDesc>& out, mm::MarchTextures& tx, int& fset, int& total, int& lanes) {
     try {  // no exception crosses the C boundary
         out.resize((size_t)n);
     } catch (...) {
@@ -767,6 +778,8 @@ static int fill_views(const mm_view* views, int32_t n, const mm_noise_set* share
         if (e) return e;
         if (fset >= 0 && fs != fset) return MM_E_FORMAT;
         fset = fs;
+        lanes = params->lanes_per_ray ? lanes_for(c, params, -1) : kDefaultViewsLanesPerRay;
+        set_lanes(c, lanes);
         if (i == 0) tx = t;
         mm::ViewDesc& d = out[(size_t)i];
         memset(&d, 0, sizeof(d));
@@ -793,14 +806,15 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
     }
     std::vector<mm::ViewDesc> h;
     mm::MarchTextures tx;
-    int fset = 0, total = 0;
-    int e = fill_views(views, n_views, shared, params, h, tx, fset, total);
+    int fset = 0, total = 0, lanes = 1;
+    int e = fill_views(views, n_views, shared, params, h, tx, fset, total, lanes);
     if (e) return e;
     reset_error_state();
     mm_view_plan* p = new (std::nothrow) mm_view_plan();
     if (!p) return MM_E_DEVICE;
     p->n_views = n_views;
     p->total_tiles = total;
+    p->lanes = lanes;
     p->fset = fset;
     p->mode = params->cone_mode;
     p->shared = tx;
@@ -839,8 +853,8 @@ int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_view
     // the shared volumes stay those of the plan; views bring new uniforms, weather maps and outputs
     std::vector<mm::ViewDesc> h;
     mm::MarchTextures tx;
-    int fset = 0, total = 0;
-    int e = fill_views(views, n_views, &plan->shared_set, params, h, tx, fset, total);
+    int fset = 0, total = 0, lanes = 1;
+    int e = fill_views(views, n_views, &plan->shared_set, params, h, tx, fset, total, lanes);
     if (e) return e;
     if (fset != plan->fset) return MM_E_FORMAT;
     reset_error_state();
@@ -850,13 +864,14 @@ int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_view
         hipStreamSynchronize(s) != hipSuccess)
         return MM_E_DEVICE;
     plan->total_tiles = total;
+    plan->lanes = lanes;
     return MM_OK;
 }
 
 int mm_view_plan_info(const mm_view_plan* plan, int32_t* total_tiles, int32_t* grid_waves) {
     if (!plan) return MM_E_NULL;
     if (total_tiles) *total_tiles = plan->total_tiles;
-    if (grid_waves) *grid_waves = 8 * ((plan->total_tiles + 7) / 8);
+    if (grid_waves) *grid_waves = plan->total_tiles;
     return MM_OK;
 }
 
@@ -864,8 +879,8 @@ int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* s
     if (!plan) return MM_E_NULL;
     if (plan->total_tiles == 0) return MM_OK;
     reset_error_state();
-    return mm::launch_views(plan->fset, plan->mode, plan->d_views, plan->n_views, plan->shared, plan->total_tiles,
-                            counts, stream);
+    return mm::launch_views(plan->fset, plan->mode, plan->lanes, plan->d_views, plan->n_views, plan->shared,
+                            plan->total_tiles, counts, stream);
 }
 
 void mm_view_plan_destroy(mm_view_plan* plan) {

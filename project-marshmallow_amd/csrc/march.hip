@@ -1269,7 +1269,7 @@ __global__ __launch_bounds__(256) void cloud_shadow_kernel(const mm::ShadowConst
 // its view by binary search over the views' first-tile indices, and marches the tile with that
 // view's constants and weather map. The hardware dispatcher hands blocks to CUs as slots free up,
 // which is the global work queue; no counter or persistent loop is needed.
-template <int FSET, bool COUNT, int MODE>
+template <int FSET, bool COUNT, int MODE, int LPR = 1>
 __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, int nViews, const MarchTextures& tx0,
                                             int total, unsigned long long* __restrict__ counts) {
     const int g = blockIdx.x;
@@ -1283,14 +1283,15 @@ __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, 
     const ViewDesc& v = views[lo];
     MarchTextures tx = tx0;
     tx.weather = v.weather;
-    march_tile<FSET, COUNT, MODE>(v.c, tx, g - v.tileBase, v.out, counts);
+    if constexpr (LPR > 1) march_rp_tile<FSET, COUNT, LPR>(v.c, tx, g - v.tileBase, v.out, counts);
+    else march_tile<FSET, COUNT, MODE>(v.c, tx, g - v.tileBase, v.out, counts);
 }
 
-template <int FSET, bool COUNT, int MODE>
+template <int FSET, bool COUNT, int MODE, int LPR>
 __global__ __launch_bounds__(64)
 void march_views_kernel(const ViewDesc* __restrict__ views, int nViews, const MarchTextures tx, int total,
                         unsigned long long* __restrict__ counts) {
-    views_group<FSET, COUNT, MODE>(views, nViews, tx, total, counts);
+    views_group<FSET, COUNT, MODE, LPR>(views, nViews, tx, total, counts);
 }
 
 template <int FSET, bool COUNT>
@@ -1301,17 +1302,22 @@ void march_views_surrogate_kernel(const ViewDesc* __restrict__ views, int nViews
 }
 
 template <int FSET, bool COUNT>
-const void* views_kernel_ptr(int mode) {
+const void* views_kernel_ptr(int mode, int lanes) {
     if (mode == CONE_SURROGATE) {
         if constexpr (FSET != mm::FSET_PREC) return (const void*)march_views_surrogate_kernel<FSET, COUNT>;
         return nullptr;
     }
-    return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN>;
+    switch (lanes) {
+        case 2: return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN, 2>;
+        case 4: return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN, 4>;
+        case 8: return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN, 8>;
+    }
+    return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN, 1>;
 }
 
 template <int FSET>
-const void* views_kernel_ptr_c(int mode, bool count) {
-    return count ? views_kernel_ptr<FSET, true>(mode) : views_kernel_ptr<FSET, false>(mode);
+const void* views_kernel_ptr_c(int mode, int lanes, bool count) {
+    return count ? views_kernel_ptr<FSET, true>(mode, lanes) : views_kernel_ptr<FSET, false>(mode, lanes);
 }
 
 template <int FSET, bool COUNT>
@@ -1355,12 +1361,13 @@ int launch_shadow(int fset, const ShadowConsts& c, const TexView& weather, const
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 
-int launch_views(int fset, int mode, const ViewDesc* views, int nViews, const MarchTextures& t, int total,
-                 unsigned long long* counts, void* stream) {
+int launch_views(int fset, int mode, int lanesPerRay, const ViewDesc* views, int nViews, const MarchTextures& t,
+                 int total, unsigned long long* counts, void* stream) {
     hipStream_t s = (hipStream_t)stream;
-    const void* k = fset == FSET_REF ? views_kernel_ptr_c<FSET_REF>(mode, counts != nullptr)
-                    : fset == FSET_FAST ? views_kernel_ptr_c<FSET_FAST>(mode, counts != nullptr)
-                                        : views_kernel_ptr_c<FSET_PREC>(mode, counts != nullptr);
+    const bool cnt = counts != nullptr;
+    const void* k = fset == FSET_REF ? views_kernel_ptr_c<FSET_REF>(mode, lanesPerRay, cnt)
+                    : fset == FSET_FAST ? views_kernel_ptr_c<FSET_FAST>(mode, lanesPerRay, cnt)
+                                        : views_kernel_ptr_c<FSET_PREC>(mode, lanesPerRay, cnt);
     if (!k) return MM_E_FORMAT;
     const unsigned grid = (unsigned)total;
     void* args[] = {(void*)&views, (void*)&nViews, (void*)&t, (void*)&total, (void*)&counts};

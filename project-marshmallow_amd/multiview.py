@@ -117,12 +117,13 @@ def time_launch(fn, reps: int = 10, warmup: int = 2) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def autotune(mm, vs: ViewSet, tiles=(8, 16, 32), steps=(0.05, 0.0625, 0.075), max_mean_err_255: float = 1.0,
-             reps: int = 5, preset: str = "scenario", base_params=None):
-    """Time every (tile width, coarse step) candidate on the view set; the coarse step changes the
-    image, so a candidate is admissible only when its mean per-channel error against the reference
-    step (steps[0]) stays within max_mean_err_255 (1/255 units, after the display tonemap).
-    Returns (best candidate, table)."""
+def autotune(mm, vs: ViewSet, shapes=((1, 8), (1, 16), (1, 32), (2, 8), (4, 8)), steps=(0.05, 0.0625, 0.075),
+             max_mean_err_255: float = 1.0, reps: int = 5, preset: str = "scenario", base_params=None):
+    """Time every (kernel shape, coarse step) candidate on the view set. A shape is (lanes per ray,
+    tile width): one lane per ray in 64-ray tiles 8, 16 or 32 wide, or the ray-parallel kernel's
+    8 x (8 / lanes) tiles; it does not change the image. The coarse step does, so a candidate is
+    admissible only when its mean per-channel error against the reference step (steps[0]) stays
+    within max_mean_err_255 (1/255 units, after the display tonemap). Returns (best candidate, table)."""
     import torch
 
     from . import training
@@ -131,9 +132,9 @@ def autotune(mm, vs: ViewSet, tiles=(8, 16, 32), steps=(0.05, 0.0625, 0.075), ma
     table = []
     for st in steps:
         outs_err = None
-        for tw in tiles:
-            prm = mm.march_params(preset, coarse_step_frac=st, tile_width=tw) if base_params is None else base_params(
-                st, tw)
+        for lanes, tw in shapes:
+            prm = mm.march_params(preset, coarse_step_frac=st, tile_width=tw, lanes_per_ray=lanes) \
+                if base_params is None else base_params(st, tw, lanes)
             plan = mm.ViewPlan(vs.views(), vs.shared, prm)
             ms = time_launch(lambda: plan.launch(), reps=reps)
             torch.cuda.synchronize()
@@ -147,7 +148,8 @@ def autotune(mm, vs: ViewSet, tiles=(8, 16, 32), steps=(0.05, 0.0625, 0.075), ma
                     err = float(np.mean(errs))
                 outs_err = err
             plan.close()
-            table.append({"tile_width": tw, "coarse_step_frac": st, "ms": ms, "mean_err_255": outs_err})
+            table.append({"lanes_per_ray": lanes, "tile_width": tw, "coarse_step_frac": st, "ms": ms,
+                          "mean_err_255": outs_err})
     ok = [r for r in table if r["mean_err_255"] <= max_mean_err_255]
     best = min(ok, key=lambda r: r["ms"])
     return best, table

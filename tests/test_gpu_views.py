@@ -1,5 +1,6 @@
 """Many views in one launch: each view's image equals its own single-view march, the
-launch replays correctly from a HIP graph, and the tile shape does not change the image."""
+launch replays correctly from a HIP graph, and neither the tile shape nor the lanes per ray change
+the image."""
 import numpy as np
 import pytest
 
@@ -24,18 +25,21 @@ def _single(mm, vs, prm):
     return outs
 
 
-@pytest.mark.parametrize("tile_width", [8, 16, 32])
-def test_views_equal_single_view_marches(mm, vs, tile_width):
+@pytest.mark.parametrize("lanes,tile_width", [(1, 8), (1, 16), (0, 32), (2, 8), (4, 8), (8, 8)])
+def test_views_equal_single_view_marches(mm, vs, lanes, tile_width):
+    """lanes 1 (and 0, the default): 64-ray tiles tile_width wide; otherwise the ray-parallel kernel's
+    8 x (8 / lanes) tiles."""
     import torch
 
-    prm = mm.march_params("scenario", tile_width=tile_width)
+    prm = mm.march_params("scenario", tile_width=tile_width, lanes_per_ray=lanes)
     ref = _single(mm, vs, mm.march_params("scenario"))
     for o in vs.outs:
         o.zero_()
     plan = mm.ViewPlan(vs.views(), vs.shared, prm)
     total, grid = plan.info()
-    assert total == sum(-(-s.width // tile_width) * -(-s.height // (64 // tile_width)) for s in vs.specs)
-    assert grid > 0
+    tw, th = (tile_width, 64 // tile_width) if lanes <= 1 else (8, 8 // lanes)
+    assert total == sum(-(-s.width // tw) * -(-s.height // th) for s in vs.specs)
+    assert grid == total
     cnt = torch.zeros(4, dtype=torch.int64, device="cuda")
     plan.launch(cnt)
     torch.cuda.synchronize()
