@@ -233,6 +233,27 @@ def test_errors_with_real_buffers(mm, gpu, tex):
     assert code == mm.MM_E_FORMAT
 
 
+@pytest.mark.parametrize("pitch", [-60.0, 5.0, 75.0, 120.0])
+def test_tile_row_order_for_camera_orientations(mm, gpu, orc, tex, pitch):
+    """The launch order of tile rows starts at the horizon (from the sign of each row's ray
+    elevation): a camera looking down (no row above the horizon), a horizon near the top, one near
+    the bottom, and an upside-down camera (pitch past 90 degrees, elevation rising down the image)
+    must all give the oracle's image, full frame and one phase, with 1 and 8 lanes per ray."""
+    W, H = 160, 120
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(pitch_deg=pitch, aspect=W / H))
+    ns = upload(mm, tex, "fast")
+    r = orc.march(cam, sun, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]),
+                  mm.march_params("scenario"), W, H)
+    for lanes in (1, 8):
+        prm = mm.march_params("scenario", lanes_per_ray=lanes)
+        g = gpu_march(mm, ns, cam, sun, sky, W, H, params=prm)
+        check_parity(g, r, f"pitch {pitch} lanes {lanes}")
+        gp = gpu_march(mm, ns, cam, sun, sky, W, H, phase=6, params=prm, fill=-1.0)
+        mask = np.zeros((H, W), bool)
+        mask[6 >> 2::4, 6 & 3::4] = True
+        assert np.array_equal(gp[mask], g[mask]) and (gp[~mask] == -1.0).all()
+
+
 def test_compute_shader_operator_ping_pong(mm, gpu, tex):
     W = H = 64
     ns = upload(mm, tex, "fast")
