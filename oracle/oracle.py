@@ -212,15 +212,20 @@ def tonemap(rgb: np.ndarray) -> np.ndarray:
 
 
 def parity_stats(gpu: np.ndarray, ref: np.ndarray) -> dict:
-    """Per-channel error after the tonemap (RGB) and raw (alpha), in units of 1/255."""
+    """Per-channel error after the tonemap (RGB) and raw (alpha), in units of 1/255, and the relative
+    error of the raw HDR RGB (|gpu - ref| / max(|ref|, 1e-6), per channel)."""
     g = np.concatenate([tonemap(gpu[..., :3]), np.asarray(gpu[..., 3:4], np.float64)], -1)
     r = np.concatenate([tonemap(ref[..., :3]), np.asarray(ref[..., 3:4], np.float64)], -1)
     err = np.abs(g - r) * 255.0
     per_px = err.reshape(-1, 4).max(1)
+    hg, hr = np.asarray(gpu[..., :3], np.float64), np.asarray(ref[..., :3], np.float64)
+    rel = np.abs(hg - hr) / np.maximum(np.abs(hr), 1e-6)
     return {
         "frac_within_1": float((per_px <= 1.0).mean()),
         "max": float(per_px.max()) if per_px.size else 0.0,
         "mean": float(err.mean()) if err.size else 0.0,
+        "hdr_rel_mean": float(rel.mean()) if rel.size else 0.0,
+        "hdr_rel_max": float(rel.max()) if rel.size else 0.0,
         "n": int(per_px.size),
         "nan": int(np.isnan(gpu).sum()),
     }

@@ -133,3 +133,26 @@ def test_series_pow_against_float64(orc):
     assert np.isnan(orc.L.orc_post_pow(-1.0, 0.45))
     assert orc.L.orc_post_pow(float("inf"), 0.45) == float("inf")
     assert abs(orc.L.orc_post_pow(1.0, 0.45) - 1.0) <= 6e-8
+
+
+def test_parity_statistics_closed_forms():
+    """The statistics the march parity tests assert on: tonemapped RGB and raw alpha errors in
+    1/255 units, and the raw HDR relative error, on perturbations whose values are known."""
+    from oracle import parity_stats, tonemap
+
+    ref = np.zeros((4, 5, 4), np.float32)
+    ref[..., :3] = 2.0
+    ref[..., 3] = 0.5
+    st = parity_stats(ref.copy(), ref)
+    assert st["max"] == 0.0 and st["frac_within_1"] == 1.0 and st["hdr_rel_max"] == 0.0 and st["n"] == 20
+    g = ref.copy()
+    g[0, 0, 3] += 3.0 / 255.0                      # alpha is compared raw: 3/255 on one pixel
+    g[1, 1, 0] *= 1.5                              # HDR red 50 % high on another
+    st = parity_stats(g, ref)
+    tm = float((tonemap(np.array([3.0])) - tonemap(np.array([2.0])))[0]) * 255.0
+    assert abs(st["max"] - max(3.0, tm)) < 1e-4
+    assert st["frac_within_1"] == (20 - 1 - (tm > 1.0)) / 20
+    assert abs(st["hdr_rel_max"] - 0.5) < 1e-7
+    assert abs(st["hdr_rel_mean"] - 0.5 / 60) < 1e-9
+    g[2, 2, 1] = np.nan
+    assert parity_stats(g, ref)["nan"] == 1
