@@ -57,9 +57,13 @@ __device__ __forceinline__ void sphere_point(I rox, I roy, I roz, I rdx, I rdy, 
     pz = (roz + rdz * t) * w + cz;
 }
 
-__device__ __forceinline__ int clamp_index(float v, int n) {
+// clamp(roundf(v), 0, n - 1) as an index (NaN gives 0), without the rounding call: clamping first
+// changes nothing (roundf is monotone and keeps the integer bounds), and in [0, n - 1] (n <= 2^15)
+// f - trunc(f) is exact, so rounding half away from zero is trunc(f) + (f - trunc(f) >= 0.5).
+__device__ __forceinline__ int round_index(float v, int n) {
     const float f = v != v ? 0.0f : fminf(fmaxf(v, 0.0f), (float)(n - 1));
-    return (int)f;
+    const int i = (int)f;
+    return i + (f - (float)i >= 0.5f ? 1 : 0);
 }
 
 __global__ __launch_bounds__(256) void reproject_kernel(const mm::ReprojConsts c, const float4* __restrict__ src,
@@ -101,15 +105,15 @@ __global__ __launch_bounds__(256) void reproject_kernel(const mm::ReprojConsts c
 #pragma unroll
     for (int s = 0; s < 10; s++) {
         const float k = __fdiv_rn((float)s, 9.0f) - 0.5f;
-        const int ix = clamp_index(roundf((oldU - bx * k) * dimx), c.width);
-        const int iy = clamp_index(roundf((oldV - by * k) * dimy), c.height);
+        const int ix = round_index((oldU - bx * k) * dimx, c.width);
+        const int iy = round_index((oldV - by * k) * dimy, c.height);
         const float4 t = src[(size_t)iy * c.pitch + ix];
         acc.x += t.x;
         acc.y += t.y;
         acc.z += t.z;
         acc.w += t.w;
     }
-    const int cx = clamp_index(roundf(oldU * dimx), c.width), cy = clamp_index(roundf(oldV * dimy), c.height);
+    const int cx = round_index(oldU * dimx, c.width), cy = round_index(oldV * dimy, c.height);
     const float a = src[(size_t)cy * c.pitch + cx].w;
     dst[(size_t)py * c.pitch + px] = make_float4(__fdiv_rn(acc.x, 10.0f), __fdiv_rn(acc.y, 10.0f),
                                                  __fdiv_rn(acc.z, 10.0f), a);
