@@ -274,6 +274,9 @@ def main():
     ms_step = start.elapsed_time(end) / args.steps
     noise_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps
     march_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps
+    step_ms_sorted = sorted(e[0].elapsed_time(e[2]) for e in ev)
+    march_ms_sorted = sorted(e[1].elapsed_time(e[2]) for e in ev)
+    median = lambda v: (v[(len(v) - 1) // 2] + v[len(v) // 2]) / 2
     assert torch.isfinite(out).all().item()
 
     b = BYTES_PER_SAMPLE[args.fset]
@@ -306,6 +309,7 @@ def main():
         "ms_per_frame": round(ms_step, 4),
         "frames_per_s": round(1000.0 / ms_step, 2),
         "march_ms": round(march_ms, 4),
+        "median_ms": {"frame": round(median(step_ms_sorted), 4), "march": round(median(march_ms_sorted), 4)},
         "noise_ms": round(noise_ms, 4),
         "density_samples_per_frame": n_density,
         "detail_samples_per_frame": n_detail,

@@ -1,0 +1,40 @@
+"""bench.py's command-line contract that needs no GPU: a multi-GPU request exits with a message
+(every box here has one GPU and this build has no multi-GPU path), and the helpers that name the
+measured kernel and read the committed HBM-traffic summary behave as documented."""
+import importlib.util
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    return b
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_kernel_name_follows_the_options():
+    b = _bench()
+    args = lambda **kw: type("A", (), dict({"no_batch": False, "lanes_per_ray": 0}, **kw))
+    assert b.march_kernel_name(args()) == "march_rp_kernel"
+    assert b.march_kernel_name(args(lanes_per_ray=1)) == "march_batched_kernel"
+    assert b.march_kernel_name(args(no_batch=True)) == "march_kernel"
+
+
+def test_traffic_comes_from_the_committed_summary():
+    b = _bench()
+    with open(os.path.join(ROOT, "profiles", "march_fast_1080p_pmc.json")) as f:
+        d = json.load(f)
+    assert b.traffic_from_profiles("fast", 1920, 1080) == d["hbm_bytes_per_launch"] > 0
+    assert b.traffic_from_profiles("fast", 640, 480) is None
