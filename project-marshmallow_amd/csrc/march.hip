@@ -1177,10 +1177,12 @@ __global__ __launch_bounds__(64) void march_rp_kernel(const MarchConsts c, const
 }
 
 #ifndef MM_SURROGATE_WAVES_PER_EU
-#define MM_SURROGATE_WAVES_PER_EU 4
+#define MM_SURROGATE_WAVES_PER_EU 6
 #endif
 // The surrogate variant bounds its register allocation: the MLP's peak (accumulators, operands)
-// otherwise sets the occupancy of the whole march.
+// otherwise sets the occupancy of the whole march. At 6 waves per SIMD the MLP spills some registers
+// to scratch and the whole launch is still the fastest (1080p, every cone replaced: 3 / 4 / 5 / 6
+// waves 2.43 / 2.24 / 2.11 / 2.10 ms).
 template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
 void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
