@@ -355,7 +355,8 @@ typedef struct mm_view_plan mm_view_plan;
 
 /* shared: curl, low_res and hi_res are used (its weather entry is ignored). params applies to every
  * view (NULL = reference preset); every view is marched with params->lanes_per_ray lanes per ray
- * (0 = 1, whatever the views' phases), with one lane in 64-ray tiles params->tile_width wide. */
+ * (0 = 1, whatever the views' phases), with one lane in 64-ray tiles params->tile_width wide
+ * (0 = 16). */
 int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_set* shared,
                         const mm_march_params* params, mm_view_plan** plan, void* stream);
 /* Replace the views' uniforms / outputs (same count) without reallocating. Synchronises the stream. */

@@ -266,6 +266,8 @@ constexpr int kDefaultPhaseLanesPerRay = 8;
 // A views launch has many more waves (64 views of 256x256 to 1920x1080: 33.6 M pixels, 1 / 2 / 4
 // lanes 20.4 / 21.7 / 29.1 ms): latency is hidden already and one lane per ray wastes no samples.
 constexpr int kDefaultViewsLanesPerRay = 1;
+// ... in 16 x 4 tiles (8 x 8 / 16 x 4 / 32 x 2: 20.4 / 20.0 / 20.0 ms).
+constexpr int kDefaultViewsTileWidth = 16;
 
 // Launch order of the tile rows: from the row just above the horizon up the sky (the longest rays
 // first), then the rows below the horizon, whose rays end at once. The sign of each tile row's ray
@@ -754,10 +756,13 @@ struct mm_view_plan {
     int device = 0;
 };
 
-// Every view of a plan is marched with the same lanes per ray: params.lanes_per_ray, or
-// kDefaultViewsLanesPerRay.
-static int fill_views(const mm_view* views, int32_t n, const mm_noise_set* shared, const mm_march_params* params,
+// Every view of a plan is marched with the same kernel shape: params.lanes_per_ray (or
+// kDefaultViewsLanesPerRay) and params.tile_width (or kDefaultViewsTileWidth).
+static int fill_views(const mm_view* views, int32_t n, const mm_noise_set* shared, const mm_march_params* given,
                       std::vector<mm::ViewDesc>& out, mm::MarchTextures& tx, int& fset, int& total, int& lanes) {
+    mm_march_params vp = *given;
+    if (vp.tile_width == 0) vp.tile_width = kDefaultViewsTileWidth;
+    const mm_march_params* params = &vp;
     try {  // no exception crosses the C boundary
         out.resize((size_t)n);
     } catch (...) {
