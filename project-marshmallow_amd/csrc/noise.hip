@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <float.h>
+#include <mutex>
 #include "mm_internal.hpp"
 #include "cr_math.hpp"
 
@@ -656,8 +657,45 @@ size_t noise_workspace_bytes() {
     return 20480 + 256 + (size_t)512 * 512 * 3 * sizeof(float);
 }
 
+namespace {
+// A second stream per device for the generators that run beside the low-res volume (the curl chain's
+// small grids, the hi-res volume and the weather map leave most CUs idle): forked from the caller's
+// stream and joined back into it with events, so the call stays ordered on that stream and can be
+// captured into a graph. Created on first use; a mutex serialises the fork / join enqueues.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+std::mutex g_side_mu;
+SideStream g_side[64];
+
+SideStream* side_stream_locked(int dev) {
+    if (dev < 0 || dev >= 64) return nullptr;
+    SideStream& d = g_side[dev];
+    if (!d.s) {
+        if (hipStreamCreateWithFlags(&d.s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        if (hipEventCreateWithFlags(&d.fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&d.join, hipEventDisableTiming) != hipSuccess)
+            return nullptr;
+    }
+    return d.join ? &d : nullptr;
+}
+}  // namespace
+
 int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream) {
-    hipStream_t s = (hipStream_t)stream;
+    hipStream_t main = (hipStream_t)stream;
+    const uint32_t side_work = p->what & (MM_GEN_CURL | MM_GEN_HI_RES | MM_GEN_WEATHER);
+    std::unique_lock<std::mutex> lock;
+    SideStream* side = nullptr;
+    if ((p->what & MM_GEN_LOW_RES) && side_work) {
+        lock = std::unique_lock<std::mutex>(g_side_mu);
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess) side = side_stream_locked(dev);
+        if (side && (hipEventRecord(side->fork, main) != hipSuccess || hipStreamWaitEvent(side->s, side->fork, 0) != hipSuccess))
+            return MM_E_DEVICE;
+        (void)hipGetLastError();  // a failed side-stream set-up leaves the work on the caller's stream
+    }
+    hipStream_t s = side ? side->s : main;
     if (p->what & MM_GEN_CURL) {
         const mm_texture& t = out->curl;
         uint8_t* ws = (uint8_t*)workspace;
@@ -673,15 +711,6 @@ int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* wor
         hipLaunchKernelGGL(curl_store_kernel, dim3(blocks_for(n)), dim3(256), 0, s, field, mmx, dim, apron, t.format,
                            (uint8_t*)t.data);
     }
-    if (p->what & MM_GEN_LOW_RES) {
-        const mm_texture& t = out->low_res;
-        GenArgs a{(uint8_t*)t.data, t.width, t.layout == MM_LAYOUT_APRON ? 1 : 0, t.format, p->seed, 0.f, {1, 1, 1, 1}};
-        if (t.format == MM_FMT_FP8X4)
-            for (int k = 0; k < 4; k++) a.inv_scale[k] = 1.0f / t.scale[k];
-        const long bricks = (long)((t.width + kBrickX - 1) / kBrickX) * ((t.width + kBrickY - 1) / kBrickY) *
-                            ((t.width + kBrickZ - 1) / kBrickZ);
-        hipLaunchKernelGGL(low_res_brick_kernel, dim3((unsigned)bricks), dim3(256), 0, s, a);
-    }
     if (p->what & MM_GEN_HI_RES) {
         const mm_texture& t = out->hi_res;
         GenArgs a{(uint8_t*)t.data, t.width, t.layout == MM_LAYOUT_APRON ? 1 : 0, t.format, p->seed, 0.f, {1, 1, 1, 1}};
@@ -695,6 +724,17 @@ int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* wor
         const long sd = t.width + a.apron;
         hipLaunchKernelGGL(weather_kernel, dim3(blocks_for(sd * sd)), dim3(256), 0, s, a);
     }
+    if (side && hipEventRecord(side->join, side->s) != hipSuccess) return MM_E_DEVICE;
+    if (p->what & MM_GEN_LOW_RES) {
+        const mm_texture& t = out->low_res;
+        GenArgs a{(uint8_t*)t.data, t.width, t.layout == MM_LAYOUT_APRON ? 1 : 0, t.format, p->seed, 0.f, {1, 1, 1, 1}};
+        if (t.format == MM_FMT_FP8X4)
+            for (int k = 0; k < 4; k++) a.inv_scale[k] = 1.0f / t.scale[k];
+        const long bricks = (long)((t.width + kBrickX - 1) / kBrickX) * ((t.width + kBrickY - 1) / kBrickY) *
+                            ((t.width + kBrickZ - 1) / kBrickZ);
+        hipLaunchKernelGGL(low_res_brick_kernel, dim3((unsigned)bricks), dim3(256), 0, main, a);
+    }
+    if (side && hipStreamWaitEvent(main, side->join, 0) != hipSuccess) return MM_E_DEVICE;
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 

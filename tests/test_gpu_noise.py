@@ -138,3 +138,36 @@ def test_weather_coverage_knob(mm, gpu, orc):
         assert (w == orc.weather(512, 2, cov)).all()
         means.append(w[..., 0].mean())
     assert means[0] < means[1] < means[2]
+
+
+def test_generation_captured_in_a_graph(mm, gpu):
+    """mm_generate_noise forks the curl, hi-res and weather generators onto a second stream beside the
+    low-res volume and joins them back with events; captured into a HIP graph, every replay (with
+    eager work and syncs in between) reproduces the eager textures byte for byte."""
+    import torch
+
+    ns = mm.NoiseSet("fast")
+    p = mm.noise_params(seed=5, weather_seed=6, coverage=0.4)
+    ns.generate(p, mm.current_stream())
+    torch.cuda.synchronize()
+    slots = ("weather", "curl", "low_res", "hi_res")
+    eager = {k: getattr(ns, k).tensor.clone() for k in slots}
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        ns.generate(p, mm.current_stream())  # warm-up outside the graph on the capture stream
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        ns.generate(p, mm.current_stream())
+    other = mm.NoiseSet("fast")
+    for rep in range(3):
+        for k in slots:
+            getattr(ns, k).tensor.zero_()
+        other.generate(mm.noise_params(seed=rep + 9), mm.current_stream())  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for k in slots:
+            assert torch.equal(getattr(ns, k).tensor, eager[k]), (rep, k)
