@@ -141,7 +141,10 @@ void mm_noise_params_default(mm_noise_params* p);
  * out->*; data must be allocated by the caller with at least mm_texture_bytes()).
  * workspace: device scratch of at least mm_noise_workspace_bytes() bytes (used by the curl
  * generator's gradient table, float curl field and global min/max), or NULL to let the library
- * allocate it stream-ordered for this call.
+ * allocate it stream-ordered for this call. With the low-res volume and any other texture selected,
+ * the other generators run on a second stream the library keeps per device, forked from `stream`
+ * and joined back into it with events: the call is ordered on `stream` like a single launch (and
+ * can be captured into a HIP graph).
  * The curl map restates ImageUtils.cpp:176-223 (GenerateCurlNoise) and does not depend on the seed.
  * Replaces: the baked TGA/PNG assets loaded by VulkanApplication.cpp:253-262 and the offline
  * curl tool GenerateCurlNoise (ImageUtils.h:12). */
