@@ -1,6 +1,6 @@
 """GPU cloud march (mm_march_clouds) against the CPU restatement of compute-clouds.comp.
 
-Bar (north star): after the reference's tonemap, >= 99.9 % of pixels within 1/255 per channel and
+Bar: after the reference's tonemap, >= 99.9 % of pixels within 1/255 per channel and
 no pixel off by more than 4/255 (alpha compared raw). Inputs: oracle-generated RGBA8 textures
 (seeded), uploaded as-is for the reference format set and repacked on the GPU for the packed sets.
 """
