@@ -253,7 +253,9 @@ int mm_reproject(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, 
 
 /* One frame of the reference's compute command buffer (VulkanApplication.cpp:1046-1071):
  * mm_reproject(source -> target) then the cloud march of the pixel phase int(sun.color.a) into
- * target. The caller swaps source and target for the next frame. */
+ * target. The caller swaps source and target for the next frame. The reprojection leaves out the
+ * phase's pixels (the march writes all of them; the image is the same) and runs beside the march on
+ * a second stream the library keeps per device, forked from and joined back into `stream`. */
 int mm_render_frame(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, const mm_sun_ubo* sun,
                     const mm_sky_ubo* sky, const mm_noise_set* textures, const mm_texture* night_sky,
                     const float* source, float* target, int32_t width, int32_t height, int32_t row_pitch_px,

@@ -525,9 +525,10 @@ int mm_march_clouds_collect(const mm_camera_ubo* camera, const mm_sun_ubo* sun, 
                       nullptr, &a, stream);
 }
 
-int mm_reproject(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, const float* source,
-                 float* target, int32_t width, int32_t height, int32_t row_pitch_px, const mm_march_params* params,
-                 void* stream) {
+// Validates one reprojection and derives its constants (every pixel written: skipPhase -1).
+static int build_reproject(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, const float* source,
+                           float* target, int32_t width, int32_t height, int32_t row_pitch_px,
+                           const mm_march_params* params, mm::ReprojConsts& c) {
     if (!camera || !camera_prev || !source || !target) return MM_E_NULL;
     if (width <= 0 || height <= 0 || width > 32768 || height > 32768) return MM_E_DIM;
     if (row_pitch_px < width) return MM_E_PITCH;
@@ -543,7 +544,6 @@ int mm_reproject(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, 
         !finite_n(camera_prev->view, 16))
         return MM_E_PARAM;
     if (!(camera->cameraParams[0] != 0.f) || !(camera->cameraParams[1] != 0.f)) return MM_E_PARAM;
-    mm::ReprojConsts c;
     memset(&c, 0, sizeof(c));
     const float* V = camera->view;
     for (int k = 0; k < 3; k++) {
@@ -561,18 +561,50 @@ int mm_reproject(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, 
     c.width = width;
     c.height = height;
     c.pitch = row_pitch_px;
+    c.skipPhase = -1;
+    return MM_OK;
+}
+
+int mm_reproject(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, const float* source,
+                 float* target, int32_t width, int32_t height, int32_t row_pitch_px, const mm_march_params* params,
+                 void* stream) {
+    mm::ReprojConsts c;
+    const int e = build_reproject(camera, camera_prev, source, target, width, height, row_pitch_px, params, c);
+    if (e) return e;
     reset_error_state();
     return mm::launch_reproject(c, source, target, stream);
 }
 
+// The reprojection writes every pixel except those of the phase the march then writes (the
+// reference's order, reproject then march, gives the same image), so the two run side by side: the
+// reprojection on the library's second stream, forked from and joined back into `stream`.
 int mm_render_frame(const mm_camera_ubo* camera, const mm_camera_ubo* camera_prev, const mm_sun_ubo* sun,
                     const mm_sky_ubo* sky, const mm_noise_set* textures, const mm_texture* night_sky,
                     const float* source, float* target, int32_t width, int32_t height, int32_t row_pitch_px,
                     const mm_march_params* params, void* stream) {
-    int e = mm_reproject(camera, camera_prev, source, target, width, height, row_pitch_px, params, stream);
+    mm::ReprojConsts rc;
+    int e = build_reproject(camera, camera_prev, source, target, width, height, row_pitch_px, params, rc);
     if (e) return e;
-    return march_impl(camera, sun, sky, textures, night_sky, target, width, height, row_pitch_px, MM_PHASE_FROM_UBO,
-                      params, nullptr, nullptr, stream);
+    mm::MarchConsts c;
+    mm::MarchTextures t;
+    int fset = 0;
+    bool empty = false;
+    e = build_march(camera, sun, sky, textures, night_sky, target, width, height, row_pitch_px, MM_PHASE_FROM_UBO,
+                    params, nullptr, c, t, fset, empty);
+    if (e) return e;
+    set_lanes(c, lanes_for(c, params, MM_PHASE_FROM_UBO));
+    if (!empty) rc.skipPhase = c.phase;
+    reset_error_state();
+    hipStream_t main = (hipStream_t)stream;
+    auto lock = mm::side_lock();
+    hipStream_t side = empty ? nullptr : mm::side_fork(main);
+    e = mm::launch_reproject(rc, source, target, side ? side : main);
+    if (!e && !empty) e = mm::launch_march(fset, c, t, target, nullptr, stream);
+    if (side) {
+        const int j = mm::side_join(main, side);
+        if (!e) e = j;
+    }
+    return e;
 }
 
 void mm_shadow_params_reference(mm_shadow_params* p) {

@@ -1,5 +1,6 @@
 // Internal declarations shared by the C-ABI layer and the HIP launchers.
 #pragma once
+#include <mutex>
 #include <stddef.h>
 #include <stdint.h>
 #include <hip/hip_runtime.h>
@@ -76,8 +77,16 @@ struct ReprojConsts {
     float centerY, innerDiam;
     int fp64;
     int width, height, pitch;
+    int skipPhase;                // -1, or the 1/16 pixel phase left to a march that follows
 };
 int launch_reproject(const ReprojConsts& c, const float* src, float* dst, void* stream);
+
+// A second stream of this device for work beside the caller's stream. side_fork (under side_lock)
+// returns it ordered after the work already on `main`, or nullptr (then use `main`); side_join makes
+// `main` wait for what was enqueued on it.
+std::unique_lock<std::mutex> side_lock();
+hipStream_t side_fork(hipStream_t main);
+int side_join(hipStream_t main, hipStream_t side);
 
 // Constants of one cloud-shadow pass (model.frag:240-283), derived on the host.
 struct ShadowConsts {

@@ -658,44 +658,63 @@ size_t noise_workspace_bytes() {
 }
 
 namespace {
-// A second stream per device for the generators that run beside the low-res volume (the curl chain's
-// small grids, the hi-res volume and the weather map leave most CUs idle): forked from the caller's
-// stream and joined back into it with events, so the call stays ordered on that stream and can be
-// captured into a graph. Created on first use; a mutex serialises the fork / join enqueues.
+// A second stream per device for work that runs beside the caller's (the curl chain's small grids,
+// the hi-res volume and the weather map beside the low-res volume; the reprojection beside a phase
+// march): forked from the caller's stream and joined back into it with events, so a call stays
+// ordered on that stream and can be captured into a graph. Created on first use; the caller holds
+// side_lock() from the fork to the join.
 struct SideStream {
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
 };
 std::mutex g_side_mu;
 SideStream g_side[64];
+}  // namespace
 
-SideStream* side_stream_locked(int dev) {
-    if (dev < 0 || dev >= 64) return nullptr;
+std::unique_lock<std::mutex> side_lock() { return std::unique_lock<std::mutex>(g_side_mu); }
+
+hipStream_t side_fork(hipStream_t main) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
     SideStream& d = g_side[dev];
     if (!d.s) {
-        if (hipStreamCreateWithFlags(&d.s, hipStreamNonBlocking) != hipSuccess) return nullptr;
-        if (hipEventCreateWithFlags(&d.fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&d.join, hipEventDisableTiming) != hipSuccess)
+        hipStream_t st = nullptr;
+        hipEvent_t f = nullptr, j = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();  // the work then stays on the caller's stream
             return nullptr;
+        }
+        d.s = st;
+        d.fork = f;
+        d.join = j;
     }
-    return d.join ? &d : nullptr;
+    if (hipEventRecord(d.fork, main) != hipSuccess || hipStreamWaitEvent(d.s, d.fork, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return d.s;
 }
-}  // namespace
+
+int side_join(hipStream_t main, hipStream_t side) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_side[dev].s != side) return MM_E_DEVICE;
+    if (hipEventRecord(g_side[dev].join, side) != hipSuccess || hipStreamWaitEvent(main, g_side[dev].join, 0) != hipSuccess)
+        return MM_E_DEVICE;
+    return MM_OK;
+}
 
 int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream) {
     hipStream_t main = (hipStream_t)stream;
     const uint32_t side_work = p->what & (MM_GEN_CURL | MM_GEN_HI_RES | MM_GEN_WEATHER);
     std::unique_lock<std::mutex> lock;
-    SideStream* side = nullptr;
+    hipStream_t side = nullptr;
     if ((p->what & MM_GEN_LOW_RES) && side_work) {
-        lock = std::unique_lock<std::mutex>(g_side_mu);
-        int dev = 0;
-        if (hipGetDevice(&dev) == hipSuccess) side = side_stream_locked(dev);
-        if (side && (hipEventRecord(side->fork, main) != hipSuccess || hipStreamWaitEvent(side->s, side->fork, 0) != hipSuccess))
-            return MM_E_DEVICE;
-        (void)hipGetLastError();  // a failed side-stream set-up leaves the work on the caller's stream
+        lock = side_lock();
+        side = side_fork(main);
     }
-    hipStream_t s = side ? side->s : main;
+    hipStream_t s = side ? side : main;
     if (p->what & MM_GEN_CURL) {
         const mm_texture& t = out->curl;
         uint8_t* ws = (uint8_t*)workspace;
@@ -724,7 +743,6 @@ int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* wor
         const long sd = t.width + a.apron;
         hipLaunchKernelGGL(weather_kernel, dim3(blocks_for(sd * sd)), dim3(256), 0, s, a);
     }
-    if (side && hipEventRecord(side->join, side->s) != hipSuccess) return MM_E_DEVICE;
     if (p->what & MM_GEN_LOW_RES) {
         const mm_texture& t = out->low_res;
         GenArgs a{(uint8_t*)t.data, t.width, t.layout == MM_LAYOUT_APRON ? 1 : 0, t.format, p->seed, 0.f, {1, 1, 1, 1}};
@@ -734,7 +752,10 @@ int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* wor
                             ((t.width + kBrickZ - 1) / kBrickZ);
         hipLaunchKernelGGL(low_res_brick_kernel, dim3((unsigned)bricks), dim3(256), 0, main, a);
     }
-    if (side && hipStreamWaitEvent(main, side->join, 0) != hipSuccess) return MM_E_DEVICE;
+    if (side) {
+        const int e = side_join(main, side);
+        if (e) return e;
+    }
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 

@@ -70,6 +70,7 @@ __global__ __launch_bounds__(256) void reproject_kernel(const mm::ReprojConsts c
                                                         float4* __restrict__ dst) {
     const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (px >= c.width || py >= c.height) return;
+    if (c.skipPhase >= 0 && (px & 3) == (c.skipPhase & 3) && (py & 3) == (c.skipPhase >> 2)) return;  // the march writes it
     const float dimx = (float)c.width, dimy = (float)c.height;
     const float uvx = __fdiv_rn((float)px, dimx), uvy = __fdiv_rn((float)py, dimy);
     const float sx = uvx * 2.0f - 1.0f, sy = uvy * 2.0f - 1.0f;

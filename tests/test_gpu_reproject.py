@@ -66,3 +66,33 @@ def test_frames_of_a_still_camera_converge_to_the_full_march(mm, gpu):
     torch.cuda.synchronize()
     a, b = img.cpu().numpy(), full.cpu().numpy()
     assert np.allclose(a, b, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("phase", [0, 6, 15])
+def test_render_frame_equals_reprojection_then_march(mm, gpu, phase):
+    """mm_render_frame runs the reprojection (minus the phase's pixels) beside the phase march on a
+    second stream: its image equals mm_reproject followed by mm_march_clouds of that phase, in the
+    reference's order, bit for bit, for a moving camera and a ragged size."""
+    import ctypes
+
+    import torch
+
+    W, H = 203, 117
+    ns = mm.NoiseSet("fast")
+    ns.generate(mm.noise_params(seed=3, weather_seed=4, coverage=0.6), mm.current_stream())
+    prm = mm.march_params("scenario")
+    cam0, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H, pitch_deg=18.0))
+    cam1, _, _ = mm.build_ubos(mm.scenario_scene(aspect=W / H, pitch_deg=20.0))
+    prev = mm.previous_camera(cam0)
+    src = torch.rand(W * H * 4, dtype=torch.float32, device="cuda")
+    sun.color[3] = float(phase)
+    a = torch.full_like(src, -5.0)
+    mm.check(mm.lib().mm_render_frame(ctypes.byref(cam1), ctypes.byref(prev), ctypes.byref(sun), ctypes.byref(sky),
+                                      ctypes.byref(ns.c), None, src.data_ptr(), a.data_ptr(), W, H, W,
+                                      ctypes.byref(prm), mm.current_stream()), "mm_render_frame")
+    b = torch.full_like(src, -5.0)
+    mm.reproject(cam1, prev, src, b, W, H, params=prm)
+    mm.march_clouds(cam1, sun, sky, ns, b, W, H, phase=phase, params=prm)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    assert not (a == -5.0).any()
