@@ -277,9 +277,32 @@ template <class T> V3<T> nightSky(const Scene<T>& s, T u, T v) {
     return v3<T>(c[0], c[1], c[2]);
 }
 
+// Optional record of one pixel's march loop, one entry per loop trip (orc_march_trace_f32):
+// t and the step size at the top of the trip, the low-res density, and what the trip did.
+enum TripEvent {
+    TRIP_COARSE_MISS = 0,   // coarse mode, density 0
+    TRIP_FIRST_HIT = 1,     // coarse mode, density > 0: back-step, fine step, uncounted continue (comp:428-433)
+    TRIP_FINE_THIN = 2,     // fine mode, detail density < 1e-4: uncounted continue (comp:437)
+    TRIP_FINE_HIT = 3,      // fine mode, light cone and accumulation (comp:440-466)
+    TRIP_FINE_MISS = 4,     // fine mode, density 0, misses below the limit
+    TRIP_REVERT = 5,        // fine mode, the miss that reaches the limit: back to coarse (comp:468-474)
+};
+struct Trace {
+    float* trips;  // [max][4]
+    int max, n;
+};
+template <class T> inline void trace_trip(Trace* tr, T t, T step, T dens, int ev) {
+    if (!tr) return;
+    if (tr->n < tr->max) {
+        float* e = tr->trips + 4 * (size_t)tr->n;
+        e[0] = (float)t; e[1] = (float)step; e[2] = (float)dens; e[3] = (float)ev;
+    }
+    tr->n++;
+}
+
 // compute-clouds.comp:288-500 for one pixel.
 template <class T>
-void marchPixel(const Scene<T>& s, int px, int py, T out[4], int* cnt) {
+void marchPixel(const Scene<T>& s, int px, int py, T out[4], int* cnt, Trace* tr = nullptr) {
     const mm_march_params* P = s.p;
     Counters<T> n{0, 0};
     T timeOffset = T(s.sky->wind[3]);
@@ -401,13 +424,19 @@ void marchPixel(const Scene<T>& s, int px, int py, T out[4], int* cnt) {
         if (density > T(0)) {
             misses = 0;
             if (noHits) {
+                trace_trip(tr, t, stepSize, density, TRIP_FIRST_HIT);
                 t -= stepSize;
                 stepSize *= T(P->fine_step_mul);
                 noHits = false;
                 continue;
             }
+            const T tripStep = stepSize;
             density = cloudHiRes<T>(s, currentPos + windOffset, stepSize, density, rHeight, n);
-            if (density < T(0.0001)) continue;
+            if (density < T(0.0001)) {
+                trace_trip(tr, t, tripStep, loDensity, TRIP_FINE_THIN);
+                continue;
+            }
+            trace_trip(tr, t, tripStep, loDensity, TRIP_FINE_HIT);
             T dal = T(0);
             for (int i = 0; i < 6; i++) {
                 V3<T> lsPos = currentPos + ((T(3) * stepSize) * samples[i]);
@@ -429,10 +458,13 @@ void marchPixel(const Scene<T>& s, int px, int py, T out[4], int* cnt) {
             accum += density;
         } else if (!noHits) {
             misses++;
+            trace_trip(tr, t, stepSize, density, misses >= P->miss_limit ? TRIP_REVERT : TRIP_FINE_MISS);
             if (misses >= P->miss_limit) {
                 noHits = true;
                 stepSize /= T(P->fine_step_mul);
             }
+        } else {
+            trace_trip(tr, t, stepSize, density, TRIP_COARSE_MISS);
         }
         if (accum > T(P->opacity_cutoff)) {
             accum = T(1);
@@ -543,6 +575,21 @@ int orc_march_pixels_f64(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const 
     return march_list<double>(cam, sun, sky, tex, prm, width, height, xy, n, out, threads);
 }
 int orc_max_threads(void) { return omp_get_max_threads(); }
+
+int orc_march_trace_f32(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                        const orc_textures* tex, const mm_march_params* prm, int width, int height, int px, int py,
+                        float* trips, int max_trips, int32_t* counts, float* out) {
+    int e = validate<float>(cam, sun, sky, tex, prm, width, height);
+    if (e) return e;
+    if (!trips || !counts || !out || max_trips < 0) return MM_E_NULL;
+    Scene<float> s{cam, sun, sky, tex, prm, width, height};
+    Trace tr{trips, max_trips, 0};
+    int c[2];
+    marchPixel<float>(s, px, py, out, c, &tr);
+    counts[0] = c[0];
+    counts[1] = c[1];
+    return tr.n;
+}
 
 }  // extern "C"
 

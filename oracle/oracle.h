@@ -41,6 +41,11 @@ void orc_set_curl_mutation(int m); /* planted mistakes for fixture-sensitivity t
 uint32_t orc_hash3(uint32_t x, uint32_t y, uint32_t z, uint32_t seed, uint32_t out[3]);
 float orc_worley(float px, float py, float pz, int cells, uint32_t seed);
 float orc_perlin_tiled(float px, float py, float pz, int period, uint32_t seed);
+float orc_worley_fbm(float px, float py, float pz, int cells, uint32_t seed);   /* cells, 2 cells, 4 cells */
+float orc_perlin_fbm(float px, float py, float pz, int period, uint32_t seed);  /* 4 octaves */
+float orc_value2(float px, float py, int period, uint32_t seed);               /* 2D value noise */
+float orc_value_fbm(float px, float py, int period, uint32_t seed);            /* 5 octaves */
+void orc_set_noise_mutation(int m); /* planted mistakes for the generator tests; 0 = none */
 void orc_low_res_texel(int dim, int x, int y, int z, uint32_t seed, float out_rgba[4]);
 void orc_hi_res_texel(int dim, int x, int y, int z, uint32_t seed, float out_rgba[4]);
 void orc_weather_texel(int dim, int x, int y, uint32_t seed, float coverage, float out_rgba[4]);
@@ -83,6 +88,15 @@ int orc_march_pixels_f64(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const 
                          const int32_t* xy, int n, double* out, int threads);
 
 int orc_max_threads(void);
+
+/* One pixel's march loop, trip by trip: trips[max_trips][4] = (t, step size, low-res density, event),
+ * event 0 coarse miss, 1 first hit (back-step, fine step, uncounted continue), 2 fine trip whose
+ * detail density is below 1e-4 (uncounted continue), 3 fine hit (light cone, accumulation), 4 fine
+ * miss, 5 the miss that reverts to coarse steps. counts = (density, detail samples); out = RGBA.
+ * Returns the number of trips (entries past max_trips are not written) or a negative MM_E_*. */
+int orc_march_trace_f32(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                        const orc_textures* tex, const mm_march_params* prm, int width, int height, int px, int py,
+                        float* trips, int max_trips, int32_t* counts, float* out);
 
 /* ---- temporal reprojection (reproject.comp) ---- */
 /* Target rows [y0, y1) from the previous frame src (both RGBA32F, pitch pixels per row). prm gives

@@ -49,6 +49,14 @@ class Oracle:
             "orc_hash3": (u32, [u32, u32, u32, u32, vp]),
             "orc_worley": (f, [f, f, f, i, u32]),
             "orc_perlin_tiled": (f, [f, f, f, i, u32]),
+            "orc_worley_fbm": (f, [f, f, f, i, u32]),
+            "orc_perlin_fbm": (f, [f, f, f, i, u32]),
+            "orc_value2": (f, [f, f, i, u32]),
+            "orc_value_fbm": (f, [f, f, i, u32]),
+            "orc_set_noise_mutation": (None, [i]),
+            "orc_low_res_texel": (None, [i, i, i, i, u32, vp]),
+            "orc_hi_res_texel": (None, [i, i, i, i, u32, vp]),
+            "orc_weather_texel": (None, [i, i, i, u32, f, vp]),
             "orc_generate_low_res": (None, [i, u32, vp]),
             "orc_generate_hi_res": (None, [i, u32, vp]),
             "orc_generate_weather": (None, [i, u32, f, vp]),
@@ -62,6 +70,7 @@ class Oracle:
             "orc_march_pixels_f32": (i, [vp, vp, vp, vp, vp, i, i, vp, i, vp, i]),
             "orc_march_pixels_f64": (i, [vp, vp, vp, vp, vp, i, i, vp, i, vp, i]),
             "orc_max_threads": (i, []),
+            "orc_march_trace_f32": (i, [vp, vp, vp, vp, vp, i, i, i, i, vp, i, vp, vp]),
             "orc_hg_phase": (f, [f, f]),
             "orc_hg_phase_f64": (d, [d, d]),
             "orc_rayleigh_phase": (f, [f]),
