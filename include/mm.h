@@ -364,7 +364,11 @@ typedef struct mm_view_plan mm_view_plan;
  * (0 = 16). */
 int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_set* shared,
                         const mm_march_params* params, mm_view_plan** plan, void* stream);
-/* Replace the views' uniforms / outputs (same count) without reallocating. Synchronises the stream. */
+/* Replace the views' uniforms / outputs (same count) without reallocating. Synchronises the stream.
+ * The launch shape must not change: an update whose views, phases, sizes or params give a different
+ * tile layout (any view's tile count) or a different lanes-per-ray returns MM_E_PARAM and leaves
+ * the plan as it was, so a graph that captured mm_march_views can be replayed after any accepted
+ * update. A different shape needs a new plan (and a new capture). */
 int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_views, const mm_march_params* params,
                         void* stream);
 /* Total tiles of the plan and the launch's grid (one wave per tile). */

@@ -785,6 +785,7 @@ struct mm_view_plan {
     mm::ViewDesc* d_views = nullptr;  // device: ViewDesc[n_views]
     mm::MarchTextures shared;
     mm_noise_set shared_set;          // the shared textures as given (curl, low_res, hi_res)
+    std::vector<int> tile_base;       // each view's first tile index (the launch's tile layout)
     int device = 0;
 };
 
@@ -856,6 +857,8 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
     p->mode = params->cone_mode;
     p->shared = tx;
     p->shared_set = *shared;
+    p->tile_base.resize((size_t)n_views);
+    for (int i = 0; i < n_views; i++) p->tile_base[(size_t)i] = h[(size_t)i].tileBase;
     (void)hipGetDevice(&p->device);
     if (fset == mm::FSET_PREC && p->mode == MM_CONE_SURROGATE) {
         delete p;
@@ -894,14 +897,17 @@ int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_view
     int e = fill_views(views, n_views, &plan->shared_set, params, h, tx, fset, total, lanes);
     if (e) return e;
     if (fset != plan->fset) return MM_E_FORMAT;
+    // the launch shape (grid = total tiles, kernel = lanes per ray) is what a captured graph keeps:
+    // an update that changes it needs a new plan, not new descriptors under an old launch
+    if (total != plan->total_tiles || lanes != plan->lanes) return MM_E_PARAM;
+    for (int i = 0; i < n_views; i++)
+        if (h[(size_t)i].tileBase != plan->tile_base[(size_t)i]) return MM_E_PARAM;
     reset_error_state();
     hipStream_t s = (hipStream_t)stream;
     if (hipMemcpyAsync(plan->d_views, h.data(), sizeof(mm::ViewDesc) * (size_t)n_views, hipMemcpyHostToDevice, s) !=
             hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return MM_E_DEVICE;
-    plan->total_tiles = total;
-    plan->lanes = lanes;
     return MM_OK;
 }
 

@@ -1283,6 +1283,9 @@ __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, 
         else hi = mid - 1;
     }
     const ViewDesc& v = views[lo];
+    // a view list whose tile counts no longer add up to the launch's total (a plan updated without a
+    // new launch shape) must not march a tile index past its view's own tiles
+    if (g - v.tileBase >= v.c.tiles) return;
     MarchTextures tx = tx0;
     tx.weather = v.weather;
     if constexpr (LPR > 1) march_rp_tile<FSET, COUNT, LPR>(v.c, tx, g - v.tileBase, v.out, counts);

@@ -658,50 +658,75 @@ size_t noise_workspace_bytes() {
 }
 
 namespace {
-// A second stream per device for work that runs beside the caller's (the curl chain's small grids,
-// the hi-res volume and the weather map beside the low-res volume; the reprojection beside a phase
-// march): forked from the caller's stream and joined back into it with events, so a call stays
-// ordered on that stream and can be captured into a graph. Created on first use; the caller holds
-// side_lock() from the fork to the join.
+// A second stream for work that runs beside the caller's (the curl chain's small grids, the hi-res
+// volume and the weather map beside the low-res volume; the reprojection beside a phase march):
+// forked from the caller's stream and joined back into it with events, so a call stays ordered on
+// that stream and can be captured into a graph. There is one side stream per (device, caller
+// stream): a side stream joins whatever capture its caller's stream is in and stays in it until that
+// capture ends, so work forked from another caller stream must never reach it. Created on first use
+// for each caller stream, up to kMaxSides per process; beyond that the work stays on the caller's
+// stream. The caller holds side_lock() from the fork to the join.
 struct SideStream {
+    int dev = -1;
+    hipStream_t main = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
 };
+constexpr int kMaxSides = 256;
 std::mutex g_side_mu;
-SideStream g_side[64];
+SideStream g_side[kMaxSides];
+int g_side_n = 0;
+
+SideStream* side_for(int dev, hipStream_t main) {
+    for (int i = 0; i < g_side_n; i++)
+        if (g_side[i].dev == dev && g_side[i].main == main) return &g_side[i];
+    if (g_side_n == kMaxSides) return nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t f = nullptr, j = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();  // the work then stays on the caller's stream
+        if (st) (void)hipStreamDestroy(st);
+        if (f) (void)hipEventDestroy(f);
+        return nullptr;
+    }
+    SideStream& d = g_side[g_side_n++];
+    d.dev = dev;
+    d.main = main;
+    d.s = st;
+    d.fork = f;
+    d.join = j;
+    return &d;
+}
 }  // namespace
 
 std::unique_lock<std::mutex> side_lock() { return std::unique_lock<std::mutex>(g_side_mu); }
 
 hipStream_t side_fork(hipStream_t main) {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-    SideStream& d = g_side[dev];
-    if (!d.s) {
-        hipStream_t st = nullptr;
-        hipEvent_t f = nullptr, j = nullptr;
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
-            (void)hipGetLastError();  // the work then stays on the caller's stream
-            return nullptr;
-        }
-        d.s = st;
-        d.fork = f;
-        d.join = j;
-    }
-    if (hipEventRecord(d.fork, main) != hipSuccess || hipStreamWaitEvent(d.s, d.fork, 0) != hipSuccess) {
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    SideStream* d = side_for(dev, main);
+    if (!d) return nullptr;
+    if (hipEventRecord(d->fork, main) != hipSuccess || hipStreamWaitEvent(d->s, d->fork, 0) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;
     }
-    return d.s;
+    return d->s;
 }
 
 int side_join(hipStream_t main, hipStream_t side) {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_side[dev].s != side) return MM_E_DEVICE;
-    if (hipEventRecord(g_side[dev].join, side) != hipSuccess || hipStreamWaitEvent(main, g_side[dev].join, 0) != hipSuccess)
+    if (hipGetDevice(&dev) != hipSuccess) return MM_E_DEVICE;
+    SideStream* d = side_for(dev, main);
+    if (!d || d->s != side) return MM_E_DEVICE;
+    if (hipEventRecord(d->join, side) != hipSuccess || hipStreamWaitEvent(main, d->join, 0) != hipSuccess) {
+        // the caller's stream could not be ordered after the side work: wait for it here, so nothing
+        // the caller enqueues next (such as freeing a workspace the side work reads) overtakes it
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(side);
         return MM_E_DEVICE;
+    }
     return MM_OK;
 }
 
