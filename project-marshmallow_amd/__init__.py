@@ -592,10 +592,11 @@ class ViewPlan:
         self.handle = h
 
     def update(self, views, params: MarchParams | None = None, stream: int | None = None) -> None:
-        self.views = (View * len(views))(*views)
+        arr = (View * len(views))(*views)
         stream = current_stream() if stream is None else stream
         pp = ctypes.byref(params) if params is not None else None
-        check(lib().mm_view_plan_update(self.handle, self.views, len(views), pp, stream), "mm_view_plan_update")
+        check(lib().mm_view_plan_update(self.handle, arr, len(views), pp, stream), "mm_view_plan_update")
+        self.views = arr
 
     def info(self):
         t, g = ctypes.c_int32(), ctypes.c_int32()

@@ -50,7 +50,12 @@ def test_views_equal_single_view_marches(mm, vs, lanes, tile_width):
 
 
 def test_views_phase_and_update(mm, vs):
+    """A phase plan writes only its phase's pixels. An update that would change the launch shape
+    (phase 5 -> every pixel: more tiles) is refused, and a same-shape update (new suns, same phase)
+    takes effect in a HIP graph captured before it."""
     import torch
+
+    from marshmallow_amd import multiview
 
     prm = mm.march_params("scenario")
     plan = mm.ViewPlan(vs.views(phase=5), vs.shared, prm)
@@ -65,10 +70,28 @@ def test_views_phase_and_update(mm, vs):
         assert (img[~m] == -1.0).all()
         if m.any():
             assert (img[m][:, 3] >= 0).all()
-    # update to all pixels: same tiles as a fresh plan
-    plan.update(vs.views(), prm)
-    t2, _ = plan.info()
-    assert t2 == mm.ViewPlan(vs.views(), vs.shared, prm).info()[0]
+    t5, _ = plan.info()
+    with pytest.raises(mm.MMError) as err:
+        plan.update(vs.views(), prm)
+    assert err.value.code == mm.MM_E_PARAM
+    assert plan.info()[0] == t5
+
+    g = multiview.capture(plan)
+    ubos = [mm.build_ubos(mm.scenario_scene(altitude=s.altitude, aspect=s.width / s.height,
+                                            sun_elevation_deg=s.sun_deg + 7.0)) for s in vs.specs]
+    plan.update([mm.make_view(*ubos[i], vs.weathers[i], vs.outs[i], s.width, s.height, phase=5)
+                 for i, s in enumerate(vs.specs)], prm)
+    for o in vs.outs:
+        o.fill_(-1.0)
+    g.replay()
+    torch.cuda.synchronize()
+    for i, s in enumerate(vs.specs):
+        ref = torch.full_like(vs.outs[i], -1.0)
+        ns = multiview._Bound(mm.NoiseSetC(vs.weathers[i].desc, vs.shared.curl.desc, vs.shared.low_res.desc,
+                                           vs.shared.hi_res.desc))
+        mm.march_clouds(*ubos[i], ns, ref, s.width, s.height, phase=5, params=prm)
+        torch.cuda.synchronize()
+        assert torch.equal(vs.outs[i], ref), (i, s)
     plan.close()
 
 
