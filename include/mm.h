@@ -151,6 +151,19 @@ void mm_noise_params_default(mm_noise_params* p);
 size_t mm_noise_workspace_bytes(const mm_noise_set* out);
 int mm_generate_noise(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream);
 
+/* Allocation-free variant of mm_generate_noise. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The curl chain's scratch lives in a fixed device array inside the library
+ * (MM_STATIC_SCRATCH_BYTES, one copy per GPU, held until the process ends); the call makes no
+ * allocation and creates no stream: every generator runs on `stream` in order.
+ * Limit: curl maps up to MM_STATIC_CURL_MAX x MM_STATIC_CURL_MAX texels (larger ones return
+ * MM_E_SIZE); other textures as mm_generate_noise. All calls on one GPU share that scratch array and
+ * it keeps the last call's contents, so calls that overlap on one GPU (other streams, other threads)
+ * race: serialise them. Results are identical to mm_generate_noise's. */
+#define MM_STATIC_CURL_MAX 256
+#define MM_STATIC_SCRATCH_BYTES (20480 + 256 + MM_STATIC_CURL_MAX * MM_STATIC_CURL_MAX * 3 * 4)
+int mm_generate_noise_noalloc(const mm_noise_params* p, const mm_noise_set* out, void* stream);
+
 /* Repack a linear RGBA8 texture (the reference's asset layout) into another format/layout.
  * kind selects the channel mapping: 0 = weather, 1 = curl, 2 = low-res, 3 = hi-res. */
 int mm_pack_texture(int32_t kind, const mm_texture* src, const mm_texture* dst, void* stream);

@@ -31,6 +31,7 @@ ERRORS = {
 FMT_RGBA8, FMT_RG8, FMT_R8E16, FMT_E16, FMT_FP8X4, FMT_FP16X4 = 1, 2, 3, 4, 5, 6
 LAYOUT_LINEAR, LAYOUT_APRON = 0, 1
 GEN_WEATHER, GEN_CURL, GEN_LOW_RES, GEN_HI_RES, GEN_ALL = 1, 2, 4, 8, 15
+STATIC_CURL_MAX = 256  # MM_STATIC_CURL_MAX: largest curl map of mm_generate_noise_noalloc
 MARCH_ISECT_FP64, MARCH_BOUNDED, MARCH_NO_BATCH = 1, 2, 4
 CONE_PLAIN, CONE_SURROGATE = 0, 1
 PHASE_ALL, PHASE_FROM_UBO = -1, -2
@@ -134,6 +135,7 @@ _SIGS = {
     "mm_noise_params_default": (None, [_P(NoiseParams)]),
     "mm_noise_workspace_bytes": (ctypes.c_size_t, [_P(NoiseSetC)]),
     "mm_generate_noise": (ctypes.c_int, [_P(NoiseParams), _P(NoiseSetC), ctypes.c_void_p, ctypes.c_void_p]),
+    "mm_generate_noise_noalloc": (ctypes.c_int, [_P(NoiseParams), _P(NoiseSetC), ctypes.c_void_p]),
     "mm_pack_texture": (ctypes.c_int, [ctypes.c_int32, _P(Texture), _P(Texture), ctypes.c_void_p]),
     "mm_march_params_reference": (None, [_P(MarchParams)]),
     "mm_march_params_scenario": (None, [_P(MarchParams)]),

@@ -207,7 +207,7 @@ size_t mm_noise_workspace_bytes(const mm_noise_set* out) {
     return mm::noise_workspace_bytes();
 }
 
-int mm_generate_noise(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream) {
+static int check_noise_call(const mm_noise_params* p, const mm_noise_set* out) {
     if (!p || !out) return MM_E_NULL;
     if (p->size != sizeof(mm_noise_params)) return MM_E_VERSION;
     if ((p->what & ~(uint32_t)MM_GEN_ALL) != 0) return MM_E_PARAM;
@@ -225,6 +225,12 @@ int mm_generate_noise(const mm_noise_params* p, const mm_noise_set* out, void* w
             out->low_res.layout != MM_LAYOUT_LINEAR)
             return MM_E_FORMAT;
     }
+    return MM_OK;
+}
+
+int mm_generate_noise(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream) {
+    const int e = check_noise_call(p, out);
+    if (e) return e;
     reset_error_state();
     void* ws = workspace;
     hipStream_t s = (hipStream_t)stream;
@@ -236,6 +242,21 @@ int mm_generate_noise(const mm_noise_params* p, const mm_noise_set* out, void* w
     int r = mm::launch_generate(p, out, ws, stream);
     if (own) (void)hipFreeAsync(ws, s);
     return r;
+}
+
+int mm_generate_noise_noalloc(const mm_noise_params* p, const mm_noise_set* out, void* stream) {
+    static_assert(MM_STATIC_SCRATCH_BYTES >= 20480 + 256, "scratch holds the gradient table and min/max cells");
+    const int e = check_noise_call(p, out);
+    if (e) return e;
+    if ((p->what & MM_GEN_CURL) && out->curl.width > MM_STATIC_CURL_MAX) return MM_E_SIZE;
+    if (mm::static_scratch_bytes() != (size_t)MM_STATIC_SCRATCH_BYTES) return MM_E_VERSION;
+    reset_error_state();
+    void* ws = nullptr;
+    if (p->what & MM_GEN_CURL) {
+        const int s = mm::static_scratch(&ws);
+        if (s) return s;
+    }
+    return mm::launch_generate(p, out, ws, stream, /*allow_side=*/false);
 }
 
 int mm_pack_texture(int32_t kind, const mm_texture* src, const mm_texture* dst, void* stream) {

@@ -140,8 +140,12 @@ int launch_march(int fset, const MarchConsts& c, const MarchTextures& t, float* 
                  void* stream);
 
 // Noise generation launchers.
-int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream);
+// allow_side = false keeps every generator on `stream` (no second stream is created or used)
+int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream,
+                    bool allow_side = true);
 size_t noise_workspace_bytes();
+size_t static_scratch_bytes();
+int static_scratch(void** ptr);
 int launch_pack(int kind, const mm_texture* src, const mm_texture* dst, void* stream);
 
 bool texture_dims_ok(const mm_texture* t, int dims);

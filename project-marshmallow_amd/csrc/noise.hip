@@ -657,6 +657,44 @@ size_t noise_workspace_bytes() {
     return 20480 + 256 + (size_t)512 * 512 * 3 * sizeof(float);
 }
 
+}  // namespace mm
+
+namespace {
+// Device scratch of the allocation-free generator entry point: the curl chain's gradient table,
+// min/max cells and float curl field for curl maps up to kStaticCurlMax^2 texels. One copy per GPU,
+// held until the process ends; every call on that GPU shares it, so calls that overlap on one GPU
+// race. The min/max cells are reset by a kernel at the start of every curl generation.
+constexpr size_t kScratchBytes = 20480 + 256 + (size_t)MM_STATIC_CURL_MAX * MM_STATIC_CURL_MAX * 3 * sizeof(float);
+__device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+namespace mm {
+
+size_t static_scratch_bytes() { return kScratchBytes; }
+
+// The device address of this GPU's copy of `scratch` (the array's name in host code is only a
+// stand-in for the symbol), looked up once per device with that device current.
+int static_scratch(void** ptr) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return MM_E_DEVICE;
+    if (dev < 0 || dev >= kMaxDevices) return MM_E_DEVICE;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return MM_E_DEVICE;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return MM_OK;
+}
+
 namespace {
 // A second stream for work that runs beside the caller's (the curl chain's small grids, the hi-res
 // volume and the weather map beside the low-res volume; the reprojection beside a phase march):
@@ -730,12 +768,13 @@ int side_join(hipStream_t main, hipStream_t side) {
     return MM_OK;
 }
 
-int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream) {
+int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* workspace, void* stream,
+                    bool allow_side) {
     hipStream_t main = (hipStream_t)stream;
     const uint32_t side_work = p->what & (MM_GEN_CURL | MM_GEN_HI_RES | MM_GEN_WEATHER);
     std::unique_lock<std::mutex> lock;
     hipStream_t side = nullptr;
-    if ((p->what & MM_GEN_LOW_RES) && side_work) {
+    if (allow_side && (p->what & MM_GEN_LOW_RES) && side_work) {
         lock = side_lock();
         side = side_fork(main);
     }
