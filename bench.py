@@ -194,6 +194,64 @@ def amortized_leg(mm, ns, cam, sun, sky, prm, W, H, warmup):
             "what": "reprojection of the previous frame + 1/16 of the pixels marched (phase cycles 0..15)"}
 
 
+SWEEP = {"coverage": (0.3, 0.5, 0.8), "sun_deg": (5.0, 25.0, 60.0), "time_s": (0.0, 150.0, 300.0, 450.0, 600.0),
+         "weather_seed": (1, 2, 3, 4, 5)}
+
+
+def sweep_leg(mm, args, prm, W, H):
+    """The 1920x1080 workload over its scene sweep (coverage x sun elevation x wind time x weather
+    seed, 225 scenes): per scene one step = every noise texture regenerated + the full-frame march,
+    timed like the headline step (HIP events, args.steps steps after args.warmup), the sample counts
+    from one instrumented launch. Prints one JSON line per scene and returns the summary."""
+    import itertools
+
+    import torch
+
+    ns = mm.NoiseSet(args.fset)
+    out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
+    stream = mm.current_stream()
+    b = BYTES_PER_SAMPLE[args.fset]
+    rows = []
+    for cov, sun_deg, t, seed in itertools.product(*SWEEP.values()):
+        cam, sun, sky = mm.build_ubos(mm.scenario_scene(altitude=200.0, pitch_deg=20.0, fov_deg=60.0, aspect=W / H,
+                                                        sun_elevation_deg=sun_deg, time=t))
+        nparams = mm.noise_params(seed=1, weather_seed=seed, coverage=cov)
+        ns.generate(nparams, stream)
+        counts = torch.zeros(4, dtype=torch.int64, device="cuda")
+        mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream, counts=counts)
+        for _ in range(args.warmup):
+            ns.generate(nparams, stream)
+            mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream)
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        for e in ev:
+            e[0].record()
+            ns.generate(nparams, stream)
+            e[1].record()
+            mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream)
+            e[2].record()
+        torch.cuda.synchronize()
+        frame = sum(e[0].elapsed_time(e[2]) for e in ev) / args.steps
+        march = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps
+        nd, nh = int(counts[0]), int(counts[1])
+        alg = nd * b["density"] + nh * b["detail"] + W * H * 16
+        row = {"coverage": cov, "sun_deg": sun_deg, "time_s": t, "weather_seed": seed, "ms_per_frame": round(frame, 4),
+               "march_ms": round(march, 4), "density_samples": nd, "detail_samples": nh,
+               "density_samples_per_s": round(nd / (frame * 1e-3), 1),
+               "roofline_hbm": {"achieved_gbs": round(alg / (march * 1e-3) / 1e9, 1),
+                                "frac": round(alg / (march * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+        rows.append(row)
+        print(json.dumps({"sweep_scene": row}), flush=True)
+    ms = sorted(r["ms_per_frame"] for r in rows)
+    sps = sorted(r["density_samples_per_s"] for r in rows)
+    fr = sorted(r["roofline_hbm"]["frac"] for r in rows)
+    pick = lambda v, q: v[min(len(v) - 1, int(q * (len(v) - 1) + 0.5))]
+    return {"scenes": len(rows), "axes": {k: list(v) for k, v in SWEEP.items()},
+            "ms_per_frame": {"min": ms[0], "median": pick(ms, 0.5), "max": ms[-1]},
+            "density_samples_per_s": {"min": sps[0], "median": pick(sps, 0.5), "max": sps[-1]},
+            "roofline_hbm_frac": {"min": fr[0], "median": pick(fr, 0.5), "max": fr[-1]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +268,8 @@ def main():
     ap.add_argument("--no-amortized", action="store_true", help="skip the 1/16-phase frame leg")
     ap.add_argument("--surrogate-steps", type=int, default=4000,
                     help="Adam steps of the learned light-cone surrogate leg (0 = skip the leg)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="also time the workload's 225-scene sweep (coverage x sun x wind time x weather seed)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -320,7 +380,10 @@ def main():
         "surrogate": None,
         "post": None,
         "amortized_frame": None,
+        "sweep": None,
     }
+    if args.sweep:
+        rec["sweep"] = sweep_leg(mm, args, prm, W, H)
     if not args.no_amortized:
         rec["amortized_frame"] = amortized_leg(mm, ns, cam, sun, sky, prm, W, H, args.warmup)
     if not args.no_post:

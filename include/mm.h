@@ -87,11 +87,17 @@ typedef struct mm_sky_ubo {
 enum {
     MM_FMT_RGBA8 = 1,       /* 4 x UNORM8, 4 B/texel: the reference format, every texture */
     MM_FMT_RG8 = 2,         /* 2 x UNORM8, 2 B/texel: weather only, (R = coverage, G = type) */
-    MM_FMT_R8E16 = 3,       /* low-res only, 4 B/texel: byte0 = R UNORM8, bytes 2-3 = 5g+2b+a (0..2040) */
-    MM_FMT_E16 = 4,         /* hi-res only, 2 B/texel: 5r+2g+b (0..2040), the 0.625/0.25/0.125 erosion sum */
+    /* 3 and 4 are not assigned */
     MM_FMT_FP8X4 = 5,       /* low-res only, 4 x FP8 e4m3 with per-channel FP32 scale (mm_texture.scale) */
     MM_FMT_FP16X4 = 6       /* 4 x FP16, 8 B/texel: curl (x,y,z,0) or hi-res (r,g,b,0) */
 };
+
+/* Texture sets the march accepts (any other combination returns MM_E_FORMAT):
+ *   REF : every texture MM_FMT_RGBA8, MM_LAYOUT_LINEAR (the reference's bindings)
+ *   FAST: MM_LAYOUT_APRON; weather MM_FMT_RG8, curl / low-res / hi-res MM_FMT_RGBA8. Same arithmetic
+ *         as REF (same image), fewer weather bytes and no index wrapping in the filters
+ *   PREC: MM_LAYOUT_APRON; weather MM_FMT_RG8, curl MM_FMT_FP16X4, low-res MM_FMT_FP8X4,
+ *         hi-res MM_FMT_FP16X4 (the precision variant) */
 
 /* Memory layouts. */
 enum {
@@ -250,6 +256,18 @@ int mm_march_clouds_counted(const mm_camera_ubo* camera, const mm_sun_ubo* sun, 
                             float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
                             int32_t phase, const mm_march_params* params,
                             unsigned long long* counts, void* stream);
+
+/* The march with each pixel's two transmittance forms written beside its colour, for error
+ * reporting against an FP64 evaluation: state_out (device, 2 floats per pixel, indexed like
+ * out_rgba32f: (y * row_pitch_px + x) * 2) receives the opacity accumulator after the horizon fade
+ * and the 0.999 cap (comp:484-486; 1 - it is the opacity complement the alpha uses, comp:496) and the
+ * light-energy accumulator `transmittance` (comp:464). Runs the lock-step kernel (the image is the
+ * same as every other kernel shape's); plain light cones only (MM_E_PARAM otherwise).
+ * No counterpart in the reference, whose shader keeps both values in registers. */
+int mm_march_clouds_state(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                          const mm_noise_set* textures, const mm_texture* night_sky,
+                          float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
+                          int32_t phase, const mm_march_params* params, float* state_out, void* stream);
 
 /* ---------------------------------------------------------------- temporal reprojection
  * Replaces the dispatch of reproject.comp (VulkanApplication.cpp:1051-1057; the shader body is

@@ -49,6 +49,57 @@ template <class T> inline T remap(T v, T a, T b, T c, T d) { return c + (((v - a
 template <class T> inline T remapClamped(T v, T a, T b, T c, T d) {
     return clampT(c + (((v - a) / (b - a)) * (d - c)), c, d);
 }
+// remap / remapClamped whose range arguments are literals of the shader. A Vulkan driver folds the
+// constant divisor into its reciprocal and multiplies (SPIR-V OpFDiv carries 2.5 ulp precision, and
+// the compilers lower x / c to x * (1 / c) with 1 / c folded once): the float restatement does the
+// same, rounding 1 / (b - a) once. The FP64 error reference keeps the quotient.
+template <class T> inline T remapK(T v, T a, T b, T c, T d) { return c + (((v - a) * (T(1) / (b - a))) * (d - c)); }
+template <> inline double remapK<double>(double v, double a, double b, double c, double d) {
+    return remap<double>(v, a, b, c, d);
+}
+template <class T> inline T remapClampedK(T v, T a, T b, T c, T d) { return clampT(remapK<T>(v, a, b, c, d), c, d); }
+
+// pow of the coverage exponent (comp:206-208). GLSL defines pow's precision as that of
+// exp2(y * log2(x)) and the drivers evaluate it with hardware log2 / exp2 whose rounding no
+// specification fixes. The restatement and the GPU kernel use one written-out evaluation, so both
+// round alike: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log2(m) = f P(f) with f = m - 1 (degree-8
+// polynomial, |error| < 3e-8, inside GLSL's 2^-21 bound for log2 on [0.5, 2]), z = y log2(x),
+// exp2(z) = 2^n Q(z - n) with n = rint(z) (degree-6 polynomial, relative error < 2e-8). pow(x, 1) = x
+// and x <= 0 gives 0. The FP64 error reference keeps the library pow.
+inline float pow_rt(float x, float y) {
+    if (y == 1.0f) return x;
+    if (!(x > 0.0f)) return 0.0f;
+    int e = 0;
+    float m = frexpf(x, &e);
+    if (m < 0.70710678f) {
+        m = m + m;
+        e = e - 1;
+    }
+    const float f = m - 1.0f;
+    float p = 1.231096834e-01f;
+    p = fmaf(p, f, -2.058618814e-01f);
+    p = fmaf(p, f, 2.160782218e-01f);
+    p = fmaf(p, f, -2.391698807e-01f);
+    p = fmaf(p, f, 2.879032493e-01f);
+    p = fmaf(p, f, -3.606932759e-01f);
+    p = fmaf(p, f, 4.809107482e-01f);
+    p = fmaf(p, f, -7.213474512e-01f);
+    p = fmaf(p, f, 1.442695022e+00f);
+    const float l = fmaf(f, p, (float)e);
+    const float z = y * l;
+    const float n = rintf(z);
+    const float r = z - n;
+    float q = 1.546144485e-04f;
+    q = fmaf(q, r, 1.340042800e-03f);
+    q = fmaf(q, r, 9.618056938e-03f);
+    q = fmaf(q, r, 5.550327152e-02f);
+    q = fmaf(q, r, 2.402265072e-01f);
+    q = fmaf(q, r, 6.931471825e-01f);
+    q = fmaf(q, r, 1.000000000e+00f);
+    return ldexpf(q, (int)n);
+}
+template <class T> inline T coveragePow(T x, T y) { return pow(x, y); }
+template <> inline float coveragePow<float>(float x, float y) { return pow_rt(x, y); }
 
 // ---- sampler (Texture.cpp:29-52, 315-338) ----
 // Unnormalised coordinate x = u * n - 0.5; with `bits` sub-texel bits it is rounded to the nearest
@@ -68,7 +119,10 @@ template <class T> inline void axis(T u, int n, int bits, int& i0, int& i1, T& f
     i0 = (int)m;
     i1 = (i0 + 1 == n) ? 0 : i0 + 1;
 }
-template <class T> inline T lerpT(T a, T b, T t) { return a + t * (b - a); }
+// Filter lerps as one fused multiply-add, a + t (b - a) rounded once. With 8-bit sub-texel weights
+// every bilinear step of UNORM8 texels is exact and the trilinear result is the exact weighted sum
+// rounded once (the last lerp's single rounding).
+template <class T> inline T lerpT(T a, T b, T t) { return fma(t, b - a, a); }
 
 // Bilinear fetch of channels [c0, c0 + nc) of an RGBA8 texture; values are filtered as the integer
 // byte values and scaled by 1/255 once (UNORM8).
@@ -193,16 +247,21 @@ Isect<I> raySphere(V3<I> ro, V3<I> rd, V3<I> c, I w) {
 template <class T> inline V3<T> projectedShellPoint(const Scene<T>& s, V3<T> pt, V3<T> center) {
     return (T(s.p->proj_radius) * normalize(pt - center)) + center;
 }
+// The shell thickness is a constant expression of the shader (comp:359): its division folds into a
+// multiplication by the reciprocal, like remapK's (the FP64 reference keeps the quotient).
 template <class T> inline T relativeHeight(V3<T> pt, V3<T> proj, T thickness) {
-    return clampT(length(pt - proj) / thickness, T(0), T(1));
+    return clampT(length(pt - proj) * (T(1) / thickness), T(0), T(1));
+}
+template <> inline double relativeHeight<double>(V3<double> pt, V3<double> proj, double thickness) {
+    return clampT(length(pt - proj) / thickness, 0.0, 1.0);
 }
 
 // compute-clouds.comp:193-204
 template <class T> T cloudLayerDensity(T h, T type) {
     h = clampT(h, T(0), T(1));
-    T cumulus = fmaxT(T(0), remap<T>(h, T(0), T(0.2), T(0), T(1)) * remap<T>(h, T(0.7), T(0.9), T(1), T(0)));
-    T strato = fmaxT(T(0), remap<T>(h, T(0), T(0.2), T(0), T(1)) * remap<T>(h, T(0.2), T(0.7), T(1), T(0)));
-    T stratus = fmaxT(T(0), remap<T>(h, T(0), T(0.1), T(0), T(1)) * remap<T>(h, T(0.2), T(0.3), T(1), T(0)));
+    T cumulus = fmaxT(T(0), remapK<T>(h, T(0), T(0.2), T(0), T(1)) * remapK<T>(h, T(0.7), T(0.9), T(1), T(0)));
+    T strato = fmaxT(T(0), remapK<T>(h, T(0), T(0.2), T(0), T(1)) * remapK<T>(h, T(0.2), T(0.7), T(1), T(0)));
+    T stratus = fmaxT(T(0), remapK<T>(h, T(0), T(0.1), T(0), T(1)) * remapK<T>(h, T(0.2), T(0.3), T(1), T(0)));
     T d1 = mixT(stratus, strato, clampT(type * T(2), T(0), T(1)));
     T d2 = mixT(strato, cumulus, clampT((type - T(0.5)) * T(2), T(0), T(1)));
     return mixT(d1, d2, type);
@@ -210,7 +269,7 @@ template <class T> T cloudLayerDensity(T h, T type) {
 
 // compute-clouds.comp:206-208
 template <class T> inline T heightBiasCoverage(T coverage, T height) {
-    return powT<T>(coverage, clampT(remap<T>(height, T(0.7), T(0.8), T(1), T(0.8)), T(0.8), T(1)));
+    return coveragePow<T>(coverage, clampT(remapK<T>(height, T(0.7), T(0.8), T(1), T(0.8)), T(0.8), T(1)));
 }
 
 template <class T> struct Counters {
@@ -246,7 +305,7 @@ T cloudTest(const Scene<T>& s, V3<T> pos, T h, V3<T> center, Counters<T>& n) {
     T layer = cloudLayerDensity<T>(h, info[2]);
     T ls = T(s.p->low_scale);
     sample3d<T>(s.tex->low, s.tex->low_dim, ls * pos.x, ls * pos.y, ls * pos.z, s.p->subtexel_bits, dn);
-    T density = layer * remapClamped<T>(dn[0], T(0.3), T(1), T(0), T(1));
+    T density = layer * remapClampedK<T>(dn[0], T(0.3), T(1), T(0), T(1));
     if (density < T(0.0001)) return T(0);
     T coverage = heightBiasCoverage<T>(h, fminT(T(0.85), info[0]));  // arguments as in comp:245
     T erosion = ((T(0.625) * dn[1]) + (T(0.25) * dn[2])) + (T(0.125) * dn[3]);
@@ -301,8 +360,10 @@ template <class T> inline void trace_trip(Trace* tr, T t, T step, T dens, int ev
 }
 
 // compute-clouds.comp:288-500 for one pixel.
+// state (optional): the opacity accumulator after the horizon fade and the 0.999 cap (comp:484-486) and
+// the light-energy accumulator `transmittance` (comp:464), the two transmittance forms of a pixel.
 template <class T>
-void marchPixel(const Scene<T>& s, int px, int py, T out[4], int* cnt, Trace* tr = nullptr) {
+void marchPixel(const Scene<T>& s, int px, int py, T out[4], int* cnt, Trace* tr = nullptr, T* state = nullptr) {
     const mm_march_params* P = s.p;
     Counters<T> n{0, 0};
     T timeOffset = T(s.sky->wind[3]);
@@ -348,6 +409,7 @@ void marchPixel(const Scene<T>& s, int px, int py, T out[4], int* cnt, Trace* tr
     if (rd.y < T(0)) {  // dot(rayDirection, vec3(0, 1, 0)) < 0
         for (int k = 0; k < 4; k++) out[k] = fc[k];
         if (cnt) { cnt[0] = 0; cnt[1] = 0; }
+        if (state) { state[0] = T(0); state[1] = T(1); }
         return;
     }
 
@@ -452,8 +514,8 @@ void marchPixel(const Scene<T>& s, int px, int py, T out[4], int* cnt, Trace* tr
             T beers = exp(-dal);
             T beersMod = fmaxT(beers, T(0.7) * exp(T(-0.25) * dal));
             beers = mixT(beers, beersMod, (-cosTheta) * T(0.5) + T(0.5));
-            T inScatter = T(0.09) + powT<T>(loDensity, remapClamped<T>(rHeight, T(0.3), T(0.85), T(0.5), T(2)));
-            inScatter *= powT<T>(remapClamped<T>(rHeight, T(0.07), T(0.34), T(0.1), T(1)), T(0.8));
+            T inScatter = T(0.09) + powT<T>(loDensity, remapClampedK<T>(rHeight, T(0.3), T(0.85), T(0.5), T(2)));
+            inScatter *= powT<T>(remapClampedK<T>(rHeight, T(0.07), T(0.34), T(0.1), T(1)), T(0.8));
             transmittance = mixT(transmittance, (inScatter * HG) * beers, T(1) - accum);
             accum += density;
         } else if (!noHits) {
@@ -474,8 +536,9 @@ void marchPixel(const Scene<T>& s, int px, int py, T out[4], int* cnt, Trace* tr
         if (!bounded && modeLimit[mode] >= 0 && ++modeCount[mode] > modeLimit[mode]) break;
     }
 
-    accum *= smoothstepT<T>(T(0), T(1), fminT(T(1), remap<T>(rd.y, T(0), T(0.1), T(0), T(1))));
+    accum *= smoothstepT<T>(T(0), T(1), fminT(T(1), remapK<T>(rd.y, T(0), T(0.1), T(0), T(1))));
     accum = fminT(accum, T(0.999));
+    if (state) { state[0] = accum; state[1] = transmittance; }
     const V3<T> sunCol = ubo3<T>(s.sun->color);
     const T I = T(s.sun->intensity);
     V3<T> cloud;
@@ -509,7 +572,7 @@ int validate(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* 
 template <class T>
 int march_rows(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky, const orc_textures* tex,
                const mm_march_params* prm, int width, int height, int phase, int y0, int y1, T* out,
-               int32_t* counts, int threads) {
+               int32_t* counts, int threads, T* state = nullptr) {
     int e = validate<T>(cam, sun, sky, tex, prm, width, height);
     if (e) return e;
     if (phase < -1 || phase > 15) return MM_E_PHASE;
@@ -524,7 +587,7 @@ int march_rows(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo
             if (phase >= 0 && (x & 3) != phase % 4) continue;
             size_t i = (size_t)y * width + x;
             int c[2];
-            marchPixel<T>(s, x, y, out + 4 * i, c);
+            marchPixel<T>(s, x, y, out + 4 * i, c, nullptr, state ? state + 2 * i : nullptr);
             if (counts) { counts[2 * i] = c[0]; counts[2 * i + 1] = c[1]; }
         }
     }
@@ -533,13 +596,16 @@ int march_rows(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo
 
 template <class T>
 int march_list(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky, const orc_textures* tex,
-               const mm_march_params* prm, int width, int height, const int32_t* xy, int n, T* out, int threads) {
+               const mm_march_params* prm, int width, int height, const int32_t* xy, int n, T* out, int threads,
+               T* state = nullptr) {
     int e = validate<T>(cam, sun, sky, tex, prm, width, height);
     if (e) return e;
     Scene<T> s{cam, sun, sky, tex, prm, width, height};
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(dynamic, 16)
-    for (int i = 0; i < n; i++) marchPixel<T>(s, xy[2 * i], xy[2 * i + 1], out + 4 * (size_t)i, nullptr);
+    for (int i = 0; i < n; i++)
+        marchPixel<T>(s, xy[2 * i], xy[2 * i + 1], out + 4 * (size_t)i, nullptr, nullptr,
+                      state ? state + 2 * (size_t)i : nullptr);
     return MM_OK;
 }
 
@@ -574,6 +640,24 @@ int orc_march_pixels_f64(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const 
                          const int32_t* xy, int n, double* out, int threads) {
     return march_list<double>(cam, sun, sky, tex, prm, width, height, xy, n, out, threads);
 }
+int orc_march_state_f32(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                        const orc_textures* tex, const mm_march_params* prm, int width, int height, float* out,
+                        float* state, int threads) {
+    if (!state) return MM_E_NULL;
+    return march_rows<float>(cam, sun, sky, tex, prm, width, height, -1, 0, height, out, nullptr, threads, state);
+}
+int orc_march_state_f64(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                        const orc_textures* tex, const mm_march_params* prm, int width, int height, double* out,
+                        double* state, int threads) {
+    if (!state) return MM_E_NULL;
+    return march_rows<double>(cam, sun, sky, tex, prm, width, height, -1, 0, height, out, nullptr, threads, state);
+}
+int orc_march_pixels_state_f64(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                               const orc_textures* tex, const mm_march_params* prm, int width, int height,
+                               const int32_t* xy, int n, double* out, double* state, int threads) {
+    if (!state) return MM_E_NULL;
+    return march_list<double>(cam, sun, sky, tex, prm, width, height, xy, n, out, threads, state);
+}
 int orc_max_threads(void) { return omp_get_max_threads(); }
 
 int orc_march_trace_f32(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
@@ -601,6 +685,7 @@ float orc_rayleigh_phase(float c) { return rayleighPhase<float>(c); }
 float orc_layer_density(float h, float type) { return cloudLayerDensity<float>(h, type); }
 float orc_height_bias_coverage(float coverage, float height) { return heightBiasCoverage<float>(coverage, height); }
 float orc_remap_clamped(float v, float a, float b, float c, float d) { return remapClamped<float>(v, a, b, c, d); }
+float orc_pow_rt(float x, float y) { return pow_rt(x, y); }
 float orc_ray_sphere_f32(const float ro[3], const float rd[3], const float c[3], float w, int* valid) {
     Isect<float> r = raySphere<float>(v3(ro[0], ro[1], ro[2]), v3(rd[0], rd[1], rd[2]), v3(c[0], c[1], c[2]), w);
     *valid = r.valid ? 1 : 0;

@@ -86,6 +86,18 @@ int orc_march_pixels_f32(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const 
 int orc_march_pixels_f64(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
                          const orc_textures* tex, const mm_march_params* prm, int width, int height,
                          const int32_t* xy, int n, double* out, int threads);
+/* Every pixel (or the listed pixels) with its two transmittance forms: state[2] per pixel = the
+ * opacity accumulator after the horizon fade and 0.999 cap (comp:484-486; 1 - state[0] is the
+ * opacity complement) and the light-energy accumulator `transmittance` (comp:464). */
+int orc_march_state_f32(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                        const orc_textures* tex, const mm_march_params* prm, int width, int height, float* out,
+                        float* state, int threads);
+int orc_march_state_f64(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                        const orc_textures* tex, const mm_march_params* prm, int width, int height, double* out,
+                        double* state, int threads);
+int orc_march_pixels_state_f64(const mm_camera_ubo* cam, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                               const orc_textures* tex, const mm_march_params* prm, int width, int height,
+                               const int32_t* xy, int n, double* out, double* state, int threads);
 
 int orc_max_threads(void);
 

@@ -70,6 +70,9 @@ class Oracle:
             "orc_march_pixels_f32": (i, [vp, vp, vp, vp, vp, i, i, vp, i, vp, i]),
             "orc_march_pixels_f64": (i, [vp, vp, vp, vp, vp, i, i, vp, i, vp, i]),
             "orc_max_threads": (i, []),
+            "orc_march_state_f32": (i, [vp, vp, vp, vp, vp, i, i, vp, vp, i]),
+            "orc_march_state_f64": (i, [vp, vp, vp, vp, vp, i, i, vp, vp, i]),
+            "orc_march_pixels_state_f64": (i, [vp, vp, vp, vp, vp, i, i, vp, i, vp, vp, i]),
             "orc_march_trace_f32": (i, [vp, vp, vp, vp, vp, i, i, i, i, vp, i, vp, vp]),
             "orc_hg_phase": (f, [f, f]),
             "orc_hg_phase_f64": (d, [d, d]),
@@ -77,6 +80,7 @@ class Oracle:
             "orc_layer_density": (f, [f, f]),
             "orc_height_bias_coverage": (f, [f, f]),
             "orc_remap_clamped": (f, [f, f, f, f, f]),
+            "orc_pow_rt": (f, [f, f]),
             "orc_ray_sphere_f32": (f, [vp, vp, vp, f, vp]),
             "orc_ray_sphere_f64": (d, [vp, vp, vp, d, vp]),
             "orc_reproject": (i, [vp, vp, vp, vp, i, i, i, vp, i, i]),
@@ -150,6 +154,27 @@ class Oracle:
         if r != 0:
             raise RuntimeError(f"oracle march failed: {r}")
         return (out, cnt) if counts else out
+
+    def march_state(self, cam, sun, sky, tex: OrcTextures, params, width: int, height: int, f64: bool = False,
+                    xy=None, threads: int = 0):
+        """RGBA and the two transmittance forms per pixel (opacity accumulator after compose, light-energy
+        accumulator), for every pixel or (FP64 only) the listed (x, y) pixels."""
+        dt = np.float64 if f64 else np.float32
+        n = height * width if xy is None else len(xy)
+        out = np.zeros((n, 4), dt)
+        state = np.zeros((n, 2), dt)
+        a = (ctypes.addressof(cam), ctypes.addressof(sun), ctypes.addressof(sky), ctypes.addressof(tex),
+             ctypes.addressof(params), width, height)
+        if xy is None:
+            fn = self.L.orc_march_state_f64 if f64 else self.L.orc_march_state_f32
+            r = fn(*a, _ptr(out), _ptr(state), threads)
+        else:
+            assert f64, "pixel lists: FP64 only"
+            xy = np.ascontiguousarray(xy, dtype=np.int32)
+            r = self.L.orc_march_pixels_state_f64(*a, _ptr(xy), len(xy), _ptr(out), _ptr(state), threads)
+        if r != 0:
+            raise RuntimeError(f"oracle march failed: {r}")
+        return out, state
 
     def reproject(self, cam, cam_prev, src: np.ndarray, params, width: int, height: int) -> np.ndarray:
         """reproject.comp restated: the target image (height, width, 4) float32 from src."""

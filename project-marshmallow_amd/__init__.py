@@ -28,19 +28,19 @@ ERRORS = {
 (MM_E_NULL, MM_E_DIM, MM_E_PHASE, MM_E_PITCH, MM_E_FORMAT, MM_E_PARAM, MM_E_DEVICE, MM_E_SIZE, MM_E_VERSION,
  MM_E_IO) = range(-1, -11, -1)
 
-FMT_RGBA8, FMT_RG8, FMT_R8E16, FMT_E16, FMT_FP8X4, FMT_FP16X4 = 1, 2, 3, 4, 5, 6
+FMT_RGBA8, FMT_RG8, FMT_FP8X4, FMT_FP16X4 = 1, 2, 5, 6
 LAYOUT_LINEAR, LAYOUT_APRON = 0, 1
 GEN_WEATHER, GEN_CURL, GEN_LOW_RES, GEN_HI_RES, GEN_ALL = 1, 2, 4, 8, 15
 STATIC_CURL_MAX = 256  # MM_STATIC_CURL_MAX: largest curl map of mm_generate_noise_noalloc
 MARCH_ISECT_FP64, MARCH_BOUNDED, MARCH_NO_BATCH = 1, 2, 4
 CONE_PLAIN, CONE_SURROGATE = 0, 1
 PHASE_ALL, PHASE_FROM_UBO = -1, -2
-TEXEL_BYTES = {FMT_RGBA8: 4, FMT_RG8: 2, FMT_R8E16: 4, FMT_E16: 2, FMT_FP8X4: 4, FMT_FP16X4: 8}
+TEXEL_BYTES = {FMT_RGBA8: 4, FMT_RG8: 2, FMT_FP8X4: 4, FMT_FP16X4: 8}
 
 # Texel format sets the march kernel runs on: (weather, curl, low-res, hi-res) formats + layout.
 FORMAT_SETS = {
     "ref": ((FMT_RGBA8, FMT_RGBA8, FMT_RGBA8, FMT_RGBA8), LAYOUT_LINEAR),
-    "fast": ((FMT_RG8, FMT_RGBA8, FMT_R8E16, FMT_E16), LAYOUT_APRON),
+    "fast": ((FMT_RG8, FMT_RGBA8, FMT_RGBA8, FMT_RGBA8), LAYOUT_APRON),
     "prec": ((FMT_RG8, FMT_FP16X4, FMT_FP8X4, FMT_FP16X4), LAYOUT_APRON),
 }
 
@@ -145,6 +145,9 @@ _SIGS = {
     "mm_march_clouds_counted": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
                                                ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                ctypes.c_int32, _P(MarchParams), ctypes.c_void_p, ctypes.c_void_p]),
+    "mm_march_clouds_state": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
+                                             ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32, _P(MarchParams), ctypes.c_void_p, ctypes.c_void_p]),
     "mm_reproject": (ctypes.c_int, [_P(CameraUBO), _P(CameraUBO), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                     ctypes.c_int32, ctypes.c_int32, _P(MarchParams), ctypes.c_void_p]),
     "mm_render_frame": (ctypes.c_int, [_P(CameraUBO), _P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC),
@@ -330,6 +333,19 @@ def march_clouds(cam: CameraUBO, sun: SunUBO, sky: SkyUBO, noise: NoiseSet, out,
                                              ctypes.byref(noise.c), nt, out.data_ptr(), width, height, pitch, phase,
                                              pp, counts.data_ptr(), stream)
         check(code, "mm_march_clouds_counted")
+
+
+def march_clouds_state(cam: CameraUBO, sun: SunUBO, sky: SkyUBO, noise: NoiseSet, out, state, width: int,
+                       height: int, phase: int = PHASE_ALL, params: MarchParams | None = None,
+                       pitch: int | None = None, stream: int | None = None) -> None:
+    """The march with each pixel's (final opacity accumulator, light-energy accumulator) written into
+    ``state`` (float32 device tensor of >= height*pitch*2 elements) beside its colour in ``out``."""
+    pitch = width if pitch is None else pitch
+    stream = current_stream() if stream is None else stream
+    pp = ctypes.byref(params) if params is not None else None
+    code = lib().mm_march_clouds_state(ctypes.byref(cam), ctypes.byref(sun), ctypes.byref(sky), ctypes.byref(noise.c),
+                                       None, out.data_ptr(), width, height, pitch, phase, pp, state.data_ptr(), stream)
+    check(code, "mm_march_clouds_state")
 
 
 class ComputeShader:

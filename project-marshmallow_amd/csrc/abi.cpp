@@ -22,8 +22,6 @@ int texel_bytes(int fmt) {
     switch (fmt) {
         case MM_FMT_RGBA8: return 4;
         case MM_FMT_RG8: return 2;
-        case MM_FMT_R8E16: return 4;
-        case MM_FMT_E16: return 2;
         case MM_FMT_FP8X4: return 4;
         case MM_FMT_FP16X4: return 8;
     }
@@ -35,8 +33,8 @@ bool slot_format_ok(int slot, int fmt) {
     switch (slot) {
         case 0: return fmt == MM_FMT_RGBA8 || fmt == MM_FMT_RG8;                              // weather
         case 1: return fmt == MM_FMT_RGBA8 || fmt == MM_FMT_FP16X4;                           // curl
-        case 2: return fmt == MM_FMT_RGBA8 || fmt == MM_FMT_R8E16 || fmt == MM_FMT_FP8X4;     // low-res
-        case 3: return fmt == MM_FMT_RGBA8 || fmt == MM_FMT_E16 || fmt == MM_FMT_FP16X4;      // hi-res
+        case 2: return fmt == MM_FMT_RGBA8 || fmt == MM_FMT_FP8X4;     // low-res
+        case 3: return fmt == MM_FMT_RGBA8 || fmt == MM_FMT_FP16X4;    // hi-res
     }
     return false;
 }
@@ -71,8 +69,8 @@ int format_set(const mm_noise_set* n) {
     if (all(MM_LAYOUT_LINEAR) && w.format == MM_FMT_RGBA8 && c.format == MM_FMT_RGBA8 && l.format == MM_FMT_RGBA8 &&
         h.format == MM_FMT_RGBA8)
         return mm::FSET_REF;
-    if (all(MM_LAYOUT_APRON) && w.format == MM_FMT_RG8 && c.format == MM_FMT_RGBA8 && l.format == MM_FMT_R8E16 &&
-        h.format == MM_FMT_E16)
+    if (all(MM_LAYOUT_APRON) && w.format == MM_FMT_RG8 && c.format == MM_FMT_RGBA8 && l.format == MM_FMT_RGBA8 &&
+        h.format == MM_FMT_RGBA8)
         return mm::FSET_FAST;
     if (all(MM_LAYOUT_APRON) && w.format == MM_FMT_RG8 && c.format == MM_FMT_FP16X4 && l.format == MM_FMT_FP8X4 &&
         h.format == MM_FMT_FP16X4)
@@ -219,11 +217,6 @@ static int check_noise_call(const mm_noise_params* p, const mm_noise_set* out) {
     if ((p->what & MM_GEN_HI_RES) && (e = check_texture(&out->hi_res, 3, 3, false))) return e;
     if (p->what & MM_GEN_CURL) {
         if (out->curl.width != out->curl.height || out->curl.width > 512) return MM_E_DIM;
-    }
-    if (p->what & MM_GEN_LOW_RES) {
-        if (out->low_res.format == MM_FMT_R8E16 && out->low_res.layout != MM_LAYOUT_APRON &&
-            out->low_res.layout != MM_LAYOUT_LINEAR)
-            return MM_E_FORMAT;
     }
     return MM_OK;
 }
@@ -504,7 +497,7 @@ static void set_lanes(mm::MarchConsts& c, int lpr) {
 static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
                       const mm_noise_set* tex, const mm_texture* night_sky, float* out, int32_t width, int32_t height,
                       int32_t pitch, int32_t phase, const mm_march_params* params, unsigned long long* counts,
-                      const CollectArgs* collect, void* stream) {
+                      const CollectArgs* collect, void* stream, float* state = nullptr) {
     mm::MarchConsts c;
     mm::MarchTextures t;
     int fset = 0;
@@ -512,6 +505,13 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
     const int e = build_march(camera, sun, sky, tex, night_sky, out, width, height, pitch, phase, params, collect, c,
                               t, fset, empty);
     if (e || empty) return e;
+    if (state) {
+        // the instrumented launch: the lock-step kernel (same image as every other kernel shape)
+        // writes each pixel's two accumulators beside its colour
+        if (c.coneMode != MM_CONE_PLAIN) return MM_E_PARAM;
+        c.flags |= MM_MARCH_NO_BATCH;
+        c.stateOut = state;
+    }
     set_lanes(c, lanes_for(c, params, phase));
     reset_error_state();
     return mm::launch_march(fset, c, t, out, counts, stream);
@@ -531,6 +531,15 @@ int mm_march_clouds_counted(const mm_camera_ubo* camera, const mm_sun_ubo* sun, 
     if (!counts) return MM_E_NULL;
     return march_impl(camera, sun, sky, textures, night_sky, out_rgba32f, width, height, row_pitch_px, phase, params,
                       counts, nullptr, stream);
+}
+
+int mm_march_clouds_state(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                          const mm_noise_set* textures, const mm_texture* night_sky, float* out_rgba32f,
+                          int32_t width, int32_t height, int32_t row_pitch_px, int32_t phase,
+                          const mm_march_params* params, float* state_out, void* stream) {
+    if (!state_out) return MM_E_NULL;
+    return march_impl(camera, sun, sky, textures, night_sky, out_rgba32f, width, height, row_pitch_px, phase, params,
+                      nullptr, nullptr, stream, state_out);
 }
 
 int mm_march_clouds_collect(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
@@ -675,7 +684,7 @@ int mm_cloud_shadow(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm
         low_res->format == MM_FMT_RGBA8)
         fset = mm::FSET_REF;
     else if (weather->layout == MM_LAYOUT_APRON && low_res->layout == MM_LAYOUT_APRON &&
-             weather->format == MM_FMT_RG8 && low_res->format == MM_FMT_R8E16)
+             weather->format == MM_FMT_RG8 && low_res->format == MM_FMT_RGBA8)
         fset = mm::FSET_FAST;
     else
         return MM_E_FORMAT;

@@ -7,11 +7,14 @@
 // REPEAT with the texture unit's fixed-point sub-texel weights (8 bits) or exact float weights,
 // over one of three texel format sets:
 //   FSET_REF : every texture RGBA8 UNORM, linear layout (the reference's bindings, drop-in)
-//   FSET_FAST: weather RG8, curl RGBA8, low-res R8 + 16-bit erosion sum, hi-res 16-bit erosion sum,
-//              apron layout (N+1 texels per axis) so a filter footprint never wraps
+//   FSET_FAST: weather RG8 (the two channels the march reads), curl / low-res / hi-res RGBA8, apron
+//              layout (N+1 texels per axis) so a filter footprint never wraps; the same arithmetic
+//              as FSET_REF, so the same image
 //   FSET_PREC: weather RG8, curl FP16x4, low-res FP8 e4m3 x4 with per-channel scale, hi-res FP16x4
-// Built with -ffp-contract=off and IEEE division/sqrt: apart from the filter lerps (FMA) and pow/exp
-// (hardware exp2/log2, GLSL precision), every float operation rounds as the oracle's restatement does.
+// Built with -ffp-contract=off and IEEE division/sqrt. Every operation that decides a ray's step
+// sequence (positions, heights, filters, layer profile, coverage pow, erosion and density remaps)
+// rounds as the oracle's restatement does, so the density samples and sample counts are the
+// restatement's bit for bit; the lighting's exp / pow use the hardware exp2 / log2 (GLSL precision).
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include "mm_internal.hpp"
@@ -48,40 +51,58 @@ __device__ __forceinline__ V3 normalize3(V3 a) { return a * __fdiv_rn(1.0f, sqrt
 __device__ __forceinline__ float length3(V3 a) { return sqrt_rn(dot3(a, a)); }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
-#ifndef MM_LERP_FMA
-#define MM_LERP_FMA 1
-#endif
-#ifndef MM_POW_ACCURATE
-#define MM_POW_ACCURATE 0
-#endif
-__device__ __forceinline__ float lerpf(float a, float b, float t) {
-#if MM_LERP_FMA
-    return __builtin_fmaf(t, b - a, a);
-#else
-    return a + t * (b - a);
-#endif
-}
-// a + t (b - a) with the product rounded: the scalar restatement's filter arithmetic.
-__device__ __forceinline__ float lerp_rounded(float a, float b, float t) { return a + t * (b - a); }
-template <bool FUSED>
-__device__ __forceinline__ float lerp_as(float a, float b, float t) {
-    if constexpr (FUSED) return lerpf(a, b, t);
-    else return lerp_rounded(a, b, t);
-}
+// Filter lerps: one fused multiply-add, a + t (b - a) rounded once. With 8-bit sub-texel weights every
+// bilinear step of UNORM8 texels is exact, so a trilinear filter returns the exact weighted sum rounded
+// once.
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
 __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
     float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
     return t * t * (3.0f - 2.0f * t);
 }
 __device__ __forceinline__ float exp2_(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float log2_(float x) { return __builtin_amdgcn_logf(x); }
-// GLSL-grade pow/exp: exp2(y * log2(x)), exp2(x * log2(e)).
-#if MM_POW_ACCURATE
-__device__ __forceinline__ float powg(float x, float y) { return powf(x, y); }
-__device__ __forceinline__ float expg(float x) { return expf(x); }
-#else
+// GLSL-grade pow/exp: exp2(y * log2(x)), exp2(x * log2(e)), with the hardware exp2 / log2.
 __device__ __forceinline__ float powg(float x, float y) { return exp2_(y * log2_(x)); }
 __device__ __forceinline__ float expg(float x) { return exp2_(x * 1.4426950408889634f); }
-#endif
+
+// pow of the coverage exponent (comp:206-208), written out so the scalar restatement rounds alike
+// (the hardware log2 / exp2 have no published rounding): x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// log2(m) = f P(f) with f = m - 1 (degree 8, |error| < 3e-8), z = y log2(x), exp2(z) = 2^n Q(z - n)
+// with n = rint(z) (degree 6, relative error < 2e-8); GLSL bounds pow by exp2(y log2 x), whose log2
+// may be off by 2^-21 on [0.5, 2]. pow(x, 1) = x; x <= 0 gives 0.
+__device__ __forceinline__ float pow_rt(float x, float y) {
+    if (y == 1.0f) return x;
+    if (!(x > 0.0f)) return 0.0f;
+    int e = 0;
+    float m = __builtin_frexpf(x, &e);
+    if (m < 0.70710678f) {
+        m = m + m;
+        e = e - 1;
+    }
+    const float f = m - 1.0f;
+    float p = 1.231096834e-01f;
+    p = __builtin_fmaf(p, f, -2.058618814e-01f);
+    p = __builtin_fmaf(p, f, 2.160782218e-01f);
+    p = __builtin_fmaf(p, f, -2.391698807e-01f);
+    p = __builtin_fmaf(p, f, 2.879032493e-01f);
+    p = __builtin_fmaf(p, f, -3.606932759e-01f);
+    p = __builtin_fmaf(p, f, 4.809107482e-01f);
+    p = __builtin_fmaf(p, f, -7.213474512e-01f);
+    p = __builtin_fmaf(p, f, 1.442695022e+00f);
+    const float l = __builtin_fmaf(f, p, (float)e);
+    const float z = y * l;
+    const float n = __builtin_rintf(z);
+    const float r = z - n;
+    float q = 1.546144485e-04f;
+    q = __builtin_fmaf(q, r, 1.340042800e-03f);
+    q = __builtin_fmaf(q, r, 9.618056938e-03f);
+    q = __builtin_fmaf(q, r, 5.550327152e-02f);
+    q = __builtin_fmaf(q, r, 2.402265072e-01f);
+    q = __builtin_fmaf(q, r, 6.931471825e-01f);
+    q = __builtin_fmaf(q, r, 1.000000000e+00f);
+    return __builtin_ldexpf(q, (int)n);
+}
+
 // remap with a constant range: (v - a) * (1 / (b - a)) * (d - c) + c
 __device__ __forceinline__ float remapk(float v, float a, float inv_ba, float c, float dc) {
     return c + (((v - a) * inv_ba) * dc);
@@ -223,33 +244,26 @@ __device__ __forceinline__ G3<BYTES> gather3(const TexView& tv, float u, float v
     return g;
 }
 
-// FUSED: the march's filter lerps (lerpf); otherwise the restatement's rounded product.
-template <bool FUSED = true, int BYTES, class F>
+template <int BYTES, class F>
 __device__ __forceinline__ float filt2(const G2<BYTES>& g, F ch) {
-    float a = lerp_as<FUSED>(ch(g.t[0]), ch(g.t[1]), g.fx);
-    float b = lerp_as<FUSED>(ch(g.t[2]), ch(g.t[3]), g.fx);
-    return lerp_as<FUSED>(a, b, g.fy);
+    float a = lerpf(ch(g.t[0]), ch(g.t[1]), g.fx);
+    float b = lerpf(ch(g.t[2]), ch(g.t[3]), g.fx);
+    return lerpf(a, b, g.fy);
 }
-template <bool FUSED = true, int BYTES, class F>
+template <int BYTES, class F>
 __device__ __forceinline__ float filt3(const G3<BYTES>& g, F ch) {
-    float a0 = lerp_as<FUSED>(ch(g.t[0]), ch(g.t[1]), g.fx);
-    float b0 = lerp_as<FUSED>(ch(g.t[2]), ch(g.t[3]), g.fx);
-    float a1 = lerp_as<FUSED>(ch(g.t[4]), ch(g.t[5]), g.fx);
-    float b1 = lerp_as<FUSED>(ch(g.t[6]), ch(g.t[7]), g.fx);
-    float s0 = lerp_as<FUSED>(a0, b0, g.fy);
-    float s1 = lerp_as<FUSED>(a1, b1, g.fy);
-    return lerp_as<FUSED>(s0, s1, g.fz);
+    float a0 = lerpf(ch(g.t[0]), ch(g.t[1]), g.fx);
+    float b0 = lerpf(ch(g.t[2]), ch(g.t[3]), g.fx);
+    float a1 = lerpf(ch(g.t[4]), ch(g.t[5]), g.fx);
+    float b1 = lerpf(ch(g.t[6]), ch(g.t[7]), g.fx);
+    float s0 = lerpf(a0, b0, g.fy);
+    float s1 = lerpf(a1, b1, g.fy);
+    return lerpf(s0, s1, g.fz);
 }
 
 // Channel decoders.
 template <int K> struct Byte {
     __device__ __forceinline__ float operator()(uint32_t v) const { return (float)((v >> (8 * K)) & 0xffu); }
-};
-struct Hi16 {
-    __device__ __forceinline__ float operator()(uint32_t v) const { return (float)(v >> 16); }
-};
-struct Lo16 {
-    __device__ __forceinline__ float operator()(uint32_t v) const { return (float)(v & 0xffffu); }
 };
 template <int K> struct Half4 {  // channel K of 4 x FP16 packed in a uint2
     __device__ __forceinline__ float operator()(uint2 v) const {
@@ -265,24 +279,23 @@ template <int K> struct Fp8 {  // channel K of 4 x FP8 e4m3 (OCP) packed in a ui
 };
 
 constexpr float kInv255 = 1.0f / 255.0f;
-constexpr float kInv2040 = 1.0f / 2040.0f;
 
 // ------------------------------------------------------------------ per-format texture reads
 struct Weather {
     float coverage, type;
 };
 
-template <int FSET, bool FUSED = true>
+template <int FSET>
 __device__ __forceinline__ Weather read_weather(const TexView& tv, float u, float v, int q8) {
     Weather w;
     if constexpr (FSET == mm::FSET_REF) {
         G2<4> g = gather2<4, false>(tv, u, v, q8);
-        w.coverage = filt2<FUSED>(g, Byte<0>()) * kInv255;
-        w.type = filt2<FUSED>(g, Byte<2>()) * kInv255;
+        w.coverage = filt2(g, Byte<0>()) * kInv255;
+        w.type = filt2(g, Byte<2>()) * kInv255;
     } else {
         G2<2> g = gather2<2, true>(tv, u, v, q8);
-        w.coverage = filt2<FUSED>(g, Byte<0>()) * kInv255;
-        w.type = filt2<FUSED>(g, Byte<1>()) * kInv255;
+        w.coverage = filt2(g, Byte<0>()) * kInv255;
+        w.type = filt2(g, Byte<1>()) * kInv255;
     }
     return w;
 }
@@ -301,26 +314,22 @@ __device__ __forceinline__ V3 read_curl(const TexView& tv, float u, float v, int
 // Hi-res erosion 0.625 r + 0.25 g + 0.125 b of the filtered texel (comp:223-224).
 template <int FSET>
 __device__ __forceinline__ float read_hi_erosion(const TexView& tv, V3 p, int q8) {
-    if constexpr (FSET == mm::FSET_REF) {
-        G3<4> g = gather3<4, false>(tv, p.x, p.y, p.z, q8);
-        float r = filt3(g, Byte<0>()) * kInv255, gg = filt3(g, Byte<1>()) * kInv255, b = filt3(g, Byte<2>()) * kInv255;
-        return ((0.625f * r) + (0.25f * gg)) + (0.125f * b);
-    } else if constexpr (FSET == mm::FSET_FAST) {
-        G3<2> g = gather3<2, true>(tv, p.x, p.y, p.z, q8);
-        return filt3(g, Lo16()) * kInv2040;
-    } else {
+    if constexpr (FSET == mm::FSET_PREC) {
         G3<8> g = gather3<8, true>(tv, p.x, p.y, p.z, q8);
         float r = filt3(g, Half4<0>()), gg = filt3(g, Half4<1>()), b = filt3(g, Half4<2>());
+        return ((0.625f * r) + (0.25f * gg)) + (0.125f * b);
+    } else {
+        G3<4> g = gather3<4, FSET != mm::FSET_REF>(tv, p.x, p.y, p.z, q8);
+        float r = filt3(g, Byte<0>()) * kInv255, gg = filt3(g, Byte<1>()) * kInv255, b = filt3(g, Byte<2>()) * kInv255;
         return ((0.625f * r) + (0.25f * gg)) + (0.125f * b);
     }
 }
 
 // Low-res read in two halves: the base shape first (so the < 1e-4 early out of comp:243 skips the
 // erosion decode), the erosion sum 0.625 g + 0.25 b + 0.125 a afterwards.
-template <int FSET> struct LowG;
-template <> struct LowG<mm::FSET_REF> { G3<4> g; };
-template <> struct LowG<mm::FSET_FAST> { G3<4> g; };
-template <> struct LowG<mm::FSET_PREC> { G3<4> g; };
+template <int FSET> struct LowG {
+    G3<4> g;
+};
 
 template <int FSET>
 __device__ __forceinline__ LowG<FSET> gather_low(const TexView& tv, V3 p, int q8) {
@@ -328,22 +337,20 @@ __device__ __forceinline__ LowG<FSET> gather_low(const TexView& tv, V3 p, int q8
     r.g = gather3<4, FSET != mm::FSET_REF>(tv, p.x, p.y, p.z, q8);
     return r;
 }
-template <int FSET, bool FUSED = true>
+template <int FSET>
 __device__ __forceinline__ float low_shape(const LowG<FSET>& l, const TexView& tv) {
-    if constexpr (FSET == mm::FSET_PREC) return filt3<FUSED>(l.g, Fp8<0>()) * tv.scale[0];
-    else return filt3<FUSED>(l.g, Byte<0>()) * kInv255;
+    if constexpr (FSET == mm::FSET_PREC) return filt3(l.g, Fp8<0>()) * tv.scale[0];
+    else return filt3(l.g, Byte<0>()) * kInv255;
 }
-template <int FSET, bool FUSED = true>
+template <int FSET>
 __device__ __forceinline__ float low_erosion(const LowG<FSET>& l, const TexView& tv) {
-    if constexpr (FSET == mm::FSET_REF) {
-        float g = filt3<FUSED>(l.g, Byte<1>()) * kInv255, b = filt3<FUSED>(l.g, Byte<2>()) * kInv255,
-              a = filt3<FUSED>(l.g, Byte<3>()) * kInv255;
+    if constexpr (FSET == mm::FSET_PREC) {
+        float g = filt3(l.g, Fp8<1>()) * tv.scale[1], b = filt3(l.g, Fp8<2>()) * tv.scale[2],
+              a = filt3(l.g, Fp8<3>()) * tv.scale[3];
         return ((0.625f * g) + (0.25f * b)) + (0.125f * a);
-    } else if constexpr (FSET == mm::FSET_FAST) {
-        return filt3<FUSED>(l.g, Hi16()) * kInv2040;
     } else {
-        float g = filt3<FUSED>(l.g, Fp8<1>()) * tv.scale[1], b = filt3<FUSED>(l.g, Fp8<2>()) * tv.scale[2],
-              a = filt3<FUSED>(l.g, Fp8<3>()) * tv.scale[3];
+        float g = filt3(l.g, Byte<1>()) * kInv255, b = filt3(l.g, Byte<2>()) * kInv255,
+              a = filt3(l.g, Byte<3>()) * kInv255;
         return ((0.625f * g) + (0.25f * b)) + (0.125f * a);
     }
 }
@@ -398,7 +405,7 @@ __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTex
     if (density < 0.0001f) return 0.0f;
     // heightBiasCoverage(relativeHeight, min(0.85, cloudInfo.r)) (comp:206-208, 245)
     float e = clampf(remapk(fminf(0.85f, w.coverage), 0.7f, 1.0f / (0.8f - 0.7f), 1.0f, 0.8f - 1.0f), 0.8f, 1.0f);
-    float coverage = powg(h, e);
+    float coverage = pow_rt(h, e);
     float erosion = low_erosion<FSET>(lg, tx.low);
     erosion = remapClamped01(erosion, coverage);
     return remapClamped01(density, erosion);
@@ -600,9 +607,14 @@ __device__ __forceinline__ RaySetup ray_setup(const MarchConsts& c, const MarchT
 }
 
 // compose (comp:484-499)
-__device__ __forceinline__ float4 compose(const MarchConsts& c, const RaySetup& r, float accum, float transmittance) {
+// The opacity accumulator after the horizon fade and the 0.999 cap (comp:484-486).
+__device__ __forceinline__ float final_accum(const RaySetup& r, float accum) {
     accum *= smoothstepf(0.0f, 1.0f, fminf(1.0f, remapk(r.rd.y, 0.0f, 1.0f / 0.1f, 0.0f, 1.0f)));
-    accum = fminf(accum, 0.999f);
+    return fminf(accum, 0.999f);
+}
+
+__device__ __forceinline__ float4 compose(const MarchConsts& c, const RaySetup& r, float accum, float transmittance) {
+    accum = final_accum(r, accum);
     const float I = c.sunIntensity;
     const float e = expg(-transmittance);
     V3 amb;
@@ -849,8 +861,10 @@ __global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const Ma
     int nd = 0, nh = 0, nj = 0, nc = 0;  // density, detail samples; cone jobs, jobs on the plain cone
     const RaySetup r = ray_setup(c, tx, px, py);
     float4* dst = reinterpret_cast<float4*>(out) + ((size_t)py * c.pitch + px);
+    float2* st = c.stateOut ? reinterpret_cast<float2*>(c.stateOut) + ((size_t)py * c.pitch + px) : nullptr;
     if (!r.march) {
         *dst = make_float4(r.bg.x, r.bg.y, r.bg.z, r.alpha);
+        if (st) *st = make_float2(0.0f, 1.0f);
         return;
     }
     MarchState m;
@@ -866,6 +880,7 @@ __global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const Ma
         }
     }
     *dst = compose(c, r, m.accum, transmittance);
+    if (st) *st = make_float2(final_accum(r, m.accum), transmittance);
     if (COUNT) {
         atomicAdd(&counts[0], (unsigned long long)nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
@@ -1216,18 +1231,18 @@ template <int FSET>
 __device__ float sh_cloud_test(const mm::ShadowConsts& c, const TexView& wt, const TexView& lt, V3 pos, float h,
                                V3 center) {
     const V3 proj = (c.projRadius * normalize3(pos - center)) + center;
-    const Weather w = read_weather<FSET, false>(wt, c.weatherScale * (proj.x - c.camPos[0]),
+    const Weather w = read_weather<FSET>(wt, c.weatherScale * (proj.x - c.camPos[0]),
                                                 c.weatherScale * (proj.z - c.camPos[2]), c.subtexelBits);
     const float layer = sh_layer(h, w.type);
     const LowG<FSET> lg = gather_low<FSET>(lt, pos * c.lowScale, c.subtexelBits);
-    float density = layer * sh_remap_clamped(low_shape<FSET, false>(lg, lt), 0.3f, 1.0f, 0.0f, 1.0f);
+    float density = layer * sh_remap_clamped(low_shape<FSET>(lg, lt), 0.3f, 1.0f, 0.0f, 1.0f);
     if (density < 0.0001f) return 0.0f;
     const float e = clampf(sh_remap(fminf(0.85f, w.coverage), 0.7f, 0.8f, 1.0f, 0.6f), 0.6f, 1.0f);
     // pow evaluated as exp2(e * log2(h)) in double and rounded once (the scalar restatement uses the
     // same formula): the shadow is not on the frame's critical path, and the single rounding keeps
     // the 1e-4 early out and the erosion remap from amplifying last-bit differences
     const float coverage = (float)exp2((double)e * log2((double)h));
-    const float erosion = sh_remap_clamped(low_erosion<FSET, false>(lg, lt), coverage, 1.0f, 0.0f, 1.0f);
+    const float erosion = sh_remap_clamped(low_erosion<FSET>(lg, lt), coverage, 1.0f, 0.0f, 1.0f);
     return sh_remap_clamped(density, erosion, 1.0f, 0.0f, 1.0f);
 }
 

@@ -465,9 +465,6 @@ __device__ __forceinline__ void low_res_values(float px, float py, float pz, con
 __device__ __forceinline__ void store_low_res(const GenArgs& a, long i, const float v[4]) {
     if (a.fmt == MM_FMT_RGBA8) {
         reinterpret_cast<uint32_t*>(a.dst)[i] = q8(v[0]) | (q8(v[1]) << 8) | (q8(v[2]) << 16) | (q8(v[3]) << 24);
-    } else if (a.fmt == MM_FMT_R8E16) {
-        uint32_t e = 5u * q8(v[1]) + 2u * q8(v[2]) + q8(v[3]);
-        reinterpret_cast<uint32_t*>(a.dst)[i] = q8(v[0]) | (e << 16);
     } else {  // MM_FMT_FP8X4
         uint32_t r = 0;
         for (int k = 0; k < 4; k++) r |= fp8_e4m3(v[k] * a.inv_scale[k]) << (8 * k);
@@ -577,8 +574,6 @@ __global__ __launch_bounds__(256) void hi_res_kernel(GenArgs a) {
     v[2] = clamp01(0.26f + 0.9f * worley_fbm(px, py, pz, 16, layer_seed(a.seed, 8)));
     if (a.fmt == MM_FMT_RGBA8) {
         reinterpret_cast<uint32_t*>(a.dst)[i] = q8(v[0]) | (q8(v[1]) << 8) | (q8(v[2]) << 16);
-    } else if (a.fmt == MM_FMT_E16) {
-        reinterpret_cast<uint16_t*>(a.dst)[i] = (uint16_t)(5u * q8(v[0]) + 2u * q8(v[1]) + q8(v[2]));
     } else {  // MM_FMT_FP16X4
         uint2 o;
         o.x = half_bits(v[0]) | (half_bits(v[1]) << 16);
@@ -627,8 +622,6 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
     switch (a.fmt) {
         case MM_FMT_RGBA8: reinterpret_cast<uint32_t*>(a.dst)[i] = t; break;
         case MM_FMT_RG8: reinterpret_cast<uint16_t*>(a.dst)[i] = (uint16_t)(c0 | (c2 << 8)); break;
-        case MM_FMT_R8E16: reinterpret_cast<uint32_t*>(a.dst)[i] = c0 | ((5u * c1 + 2u * c2 + c3) << 16); break;
-        case MM_FMT_E16: reinterpret_cast<uint16_t*>(a.dst)[i] = (uint16_t)(5u * c0 + 2u * c1 + c2); break;
         case MM_FMT_FP16X4: {
             uint2 o;
             o.x = half_bits((float)c0 / 255.0f) | (half_bits((float)c1 / 255.0f) << 16);

@@ -27,7 +27,7 @@ static int check(void) {
     EXPECT(strcmp(mm_strerror(MM_OK), "ok") == 0);
     EXPECT(mm_strerror(MM_E_IO) != NULL);
     EXPECT(mm_texture_bytes(128, 128, 128, MM_FMT_RGBA8, MM_LAYOUT_LINEAR) == (size_t)128 * 128 * 128 * 4);
-    EXPECT(mm_texture_bytes(32, 32, 32, MM_FMT_E16, MM_LAYOUT_APRON) == (size_t)33 * 33 * 33 * 2);
+    EXPECT(mm_texture_bytes(32, 32, 32, MM_FMT_RGBA8, MM_LAYOUT_APRON) == (size_t)33 * 33 * 33 * 4);
     EXPECT(sizeof(mm_camera_ubo) == 160 && sizeof(mm_sun_ubo) == 128 && sizeof(mm_sky_ubo) == 64);
     mm_march_params p;
     mm_march_params_reference(&p);
@@ -67,8 +67,8 @@ static int run(int W, int H, const char* path) {
     mm_noise_set set;
     EXPECT(!texture(&set.weather, 512, 512, 1, MM_FMT_RG8, MM_LAYOUT_APRON));
     EXPECT(!texture(&set.curl, 128, 128, 1, MM_FMT_RGBA8, MM_LAYOUT_APRON));
-    EXPECT(!texture(&set.low_res, 128, 128, 128, MM_FMT_R8E16, MM_LAYOUT_APRON));
-    EXPECT(!texture(&set.hi_res, 32, 32, 32, MM_FMT_E16, MM_LAYOUT_APRON));
+    EXPECT(!texture(&set.low_res, 128, 128, 128, MM_FMT_RGBA8, MM_LAYOUT_APRON));
+    EXPECT(!texture(&set.hi_res, 32, 32, 32, MM_FMT_RGBA8, MM_LAYOUT_APRON));
     mm_noise_params np;
     mm_noise_params_default(&np);
     np.seed = 1;

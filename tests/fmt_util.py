@@ -5,7 +5,7 @@ values (nearest, ties to even code) rather than bit manipulation.
 """
 import numpy as np
 
-FMT_RGBA8, FMT_RG8, FMT_R8E16, FMT_E16, FMT_FP8X4, FMT_FP16X4 = 1, 2, 3, 4, 5, 6
+FMT_RGBA8, FMT_RG8, FMT_FP8X4, FMT_FP16X4 = 1, 2, 5, 6
 LAYOUT_LINEAR, LAYOUT_APRON = 0, 1
 
 
@@ -59,10 +59,6 @@ def pack_curl(rgba: np.ndarray, layout: int) -> bytes:
 def pack_low(rgba: np.ndarray, fmt: int, layout: int, f32: np.ndarray | None = None, inv_scale=None) -> bytes:
     if fmt == FMT_RGBA8:
         a = rgba
-    elif fmt == FMT_R8E16:
-        c = rgba.astype(np.uint32)
-        e = 5 * c[..., 1] + 2 * c[..., 2] + c[..., 3]
-        a = (c[..., 0] | (e << 16)).astype(np.uint32)
     elif fmt == FMT_FP8X4:
         a = fp8_e4m3_encode(np.asarray(f32, np.float64) * np.asarray(inv_scale, np.float64))
     else:
@@ -75,9 +71,6 @@ def pack_low(rgba: np.ndarray, fmt: int, layout: int, f32: np.ndarray | None = N
 def pack_hi(rgba: np.ndarray, fmt: int, layout: int, f32: np.ndarray | None = None) -> bytes:
     if fmt == FMT_RGBA8:
         a = rgba
-    elif fmt == FMT_E16:
-        c = rgba.astype(np.uint32)
-        a = (5 * c[..., 0] + 2 * c[..., 1] + c[..., 2]).astype(np.uint16)
     elif fmt == FMT_FP16X4:
         h = np.zeros(f32.shape, np.float16)
         h[..., :3] = f32[..., :3].astype(np.float16)

@@ -33,8 +33,9 @@ def test_version_and_errors(mm):
 
 def test_texture_bytes(mm):
     assert mm.texture_bytes(128, 128, 128, mm.FMT_RGBA8, mm.LAYOUT_LINEAR) == 128 ** 3 * 4
-    assert mm.texture_bytes(128, 128, 128, mm.FMT_R8E16, mm.LAYOUT_APRON) == 129 ** 3 * 4
-    assert mm.texture_bytes(32, 32, 32, mm.FMT_E16, mm.LAYOUT_APRON) == 33 ** 3 * 2
+    assert mm.texture_bytes(128, 128, 128, mm.FMT_RGBA8, mm.LAYOUT_APRON) == 129 ** 3 * 4
+    assert mm.texture_bytes(32, 32, 32, mm.FMT_RGBA8, mm.LAYOUT_APRON) == 33 ** 3 * 4
+    assert mm.texture_bytes(32, 32, 32, 4, mm.LAYOUT_APRON) == 0  # format codes 3 and 4 are not assigned
     assert mm.texture_bytes(512, 512, 1, mm.FMT_RG8, mm.LAYOUT_APRON) == 513 * 513 * 2
     assert mm.texture_bytes(128, 128, 1, mm.FMT_FP16X4, mm.LAYOUT_APRON) == 129 * 129 * 8
     assert mm.texture_bytes(8, 8, 1, 99, mm.LAYOUT_LINEAR) == 0
@@ -128,8 +129,8 @@ def test_march_rejects_bad_textures(mm, cpu_only):
     assert _march(mm, _noise(mm, weather=_FakeTex(mm, 500, 500, 1, mm.FMT_RGBA8, 0))) == mm.MM_E_DIM
     assert _march(mm, _noise(mm, low_res=_FakeTex(mm, 128, 128, 64, mm.FMT_RGBA8, 0))) == mm.MM_E_DIM
     assert _march(mm, _noise(mm, hi_res=_FakeTex(mm, 32, 32, 32, mm.FMT_RGBA8, 0, nbytes=100))) == mm.MM_E_SIZE
-    assert _march(mm, _noise(mm, hi_res=_FakeTex(mm, 32, 32, 32, mm.FMT_E16, mm.LAYOUT_APRON))) == mm.MM_E_FORMAT
-    assert _march(mm, _noise(mm, curl=_FakeTex(mm, 128, 128, 1, mm.FMT_R8E16, 0))) == mm.MM_E_FORMAT
+    assert _march(mm, _noise(mm, hi_res=_FakeTex(mm, 32, 32, 32, mm.FMT_RGBA8, mm.LAYOUT_APRON))) == mm.MM_E_FORMAT
+    assert _march(mm, _noise(mm, curl=_FakeTex(mm, 128, 128, 1, mm.FMT_FP8X4, 0))) == mm.MM_E_FORMAT
     # a volume of 4 GiB or more (1024^3 RGBA8 = exactly 4 GiB): the kernels use 32-bit byte offsets
     assert _march(mm, _noise(mm, low_res=_FakeTex(mm, 1024, 1024, 1024, mm.FMT_RGBA8, 0))) == mm.MM_E_DIM
     n = _noise(mm)
@@ -151,9 +152,9 @@ def test_generate_rejects_bad_arguments(mm, cpu_only):
     assert lib.mm_generate_noise(ctypes.byref(bad), ctypes.byref(n), None, None) == mm.MM_E_PARAM
     n2 = _noise(mm, curl=_FakeTex(mm, 1024, 1024, 1, mm.FMT_RGBA8, 0))
     assert lib.mm_generate_noise(ctypes.byref(p), ctypes.byref(n2), None, None) == mm.MM_E_DIM
-    n3 = _noise(mm, low_res=_FakeTex(mm, 128, 128, 128, mm.FMT_E16, 0))
+    n3 = _noise(mm, low_res=_FakeTex(mm, 128, 128, 128, mm.FMT_FP16X4, 0))
     assert lib.mm_generate_noise(ctypes.byref(p), ctypes.byref(n3), None, None) == mm.MM_E_FORMAT
     src = _FakeTex(mm, 32, 32, 32, mm.FMT_RGBA8, 0)
-    dst = _FakeTex(mm, 16, 16, 16, mm.FMT_E16, mm.LAYOUT_APRON)
+    dst = _FakeTex(mm, 16, 16, 16, mm.FMT_RGBA8, mm.LAYOUT_APRON)
     assert lib.mm_pack_texture(3, ctypes.byref(src.desc), ctypes.byref(dst.desc), None) == mm.MM_E_DIM
     assert lib.mm_pack_texture(9, ctypes.byref(src.desc), ctypes.byref(dst.desc), None) == mm.MM_E_PARAM

@@ -14,6 +14,10 @@ from oracle import parity_stats
 pytestmark = pytest.mark.gpu
 
 TOL_FRAC, TOL_MAX = 0.999, 4.0
+# Every operation that decides a ray's steps rounds like the restatement, so only the lighting's
+# exp / pow (hardware exp2 / log2 against the restatement's libm) differ, and smoothly: measured below
+# 2e-4 / 255 on every pixel of the 1080p and 4K frames. Checked on top of the bar.
+TOL_EXACT_PATH = 0.01
 
 
 @pytest.fixture(scope="module")
@@ -50,12 +54,14 @@ def gpu_march(mm, ns, cam, sun, sky, W, H, phase=-1, params=None, night=None, fi
     return (img, cnt.cpu().numpy()) if counts else img
 
 
-def check_parity(g, r, what):
+def check_parity(g, r, what, exact_path=True):
     st = parity_stats(g, r)
     print(what, st, flush=True)
     assert st["nan"] == 0
     assert st["frac_within_1"] >= TOL_FRAC, (what, st)
     assert st["max"] <= TOL_MAX, (what, st)
+    if exact_path:
+        assert st["max"] <= TOL_EXACT_PATH, (what, st)
     return st
 
 
@@ -95,22 +101,18 @@ def test_parity_variants(mm, gpu, orc, tex, case):
 
 
 @pytest.mark.parametrize("preset", ["reference", "scenario"])
-def test_packed_set_deviation(mm, gpu, orc, tex, preset):
-    """The FAST set filters the exact integer erosion sum 5g + 2b + a (and reads two weather channels)
-    instead of three filtered channels: equal in exact arithmetic, different in the last float bits.
-    The march's density thresholds (1e-4 early outs, the first-hit back step, 10-miss revert) turn
-    such last-bit differences into a different step sequence on a few rays, so the packed set meets
-    the 1/255 fraction but not the 4/255 maximum everywhere: measured max 10.6/255 on 5 of 36864
-    pixels (reference preset, FP32 intersection). The bar here: >= 99.9 % within 1/255, mean below
-    0.01/255, max reported."""
+def test_fast_set_is_the_reference_set_image(mm, gpu, orc, tex, preset):
+    """The FAST set stores the same UNORM8 channels in the apron layout (and only the two weather
+    channels the march reads) and filters them with the same arithmetic: image and sample counts
+    equal the reference set's bit for bit, and both meet the parity bar against the oracle."""
     W = H = 192
     prm = mm.march_params(preset)
     cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=1.0))
-    g = gpu_march(mm, upload(mm, tex, "fast"), cam, sun, sky, W, H, params=prm)
+    g, cg = gpu_march(mm, upload(mm, tex, "fast"), cam, sun, sky, W, H, params=prm, counts=True)
+    h, ch = gpu_march(mm, upload(mm, tex, "ref"), cam, sun, sky, W, H, params=prm, counts=True)
+    assert np.array_equal(g, h) and (cg == ch).all()
     r = orc.march(cam, sun, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]), prm, W, H)
-    st = parity_stats(g, r)
-    print("packed", preset, st, flush=True)
-    assert st["nan"] == 0 and st["frac_within_1"] >= TOL_FRAC and st["mean"] < 0.01
+    check_parity(g, r, f"fast {preset}")
 
 
 def test_night_branch(mm, gpu, orc, tex):
@@ -154,17 +156,19 @@ def test_phase_from_ubo(mm, gpu, tex):
     assert np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(np.nan_to_num(a), np.nan_to_num(b))
 
 
-def test_sample_counts_match_oracle(mm, gpu, orc, tex):
+@pytest.mark.parametrize("preset", ["reference", "scenario"])
+def test_sample_counts_match_oracle(mm, gpu, orc, tex, preset):
+    """Every operation that decides a ray's steps rounds like the restatement: the kernel takes
+    exactly the oracle's density and detail samples."""
     W = H = 128
     cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=1.0))
-    prm = mm.march_params("scenario")
+    prm = mm.march_params(preset)
     _, cnt = gpu_march(mm, upload(mm, tex, "fast"), cam, sun, sky, W, H, params=prm, counts=True)
     _, rc = orc.march(cam, sun, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]), prm, W, H,
                       counts=True)
     d_ref, h_ref = int(rc[..., 0].sum()), int(rc[..., 1].sum())
     assert d_ref > 0 and h_ref > 0
-    assert abs(int(cnt[0]) - d_ref) <= 0.002 * d_ref
-    assert abs(int(cnt[1]) - h_ref) <= 0.002 * h_ref
+    assert int(cnt[0]) == d_ref and int(cnt[1]) == h_ref
 
 
 def test_ray_invariants(mm, gpu, orc, tex):
@@ -190,22 +194,23 @@ def test_ray_invariants(mm, gpu, orc, tex):
     assert (g2[top, :, 3] <= 0.0011 * g[top, :, 3] + 1e-7).all()
 
 
-def test_1080p_properties_and_sampled_parity(mm, gpu, orc, tex):
-    W, H = 1920, 1080
+@pytest.mark.parametrize("W,H", [(1920, 1080), (3840, 2160)])
+def test_full_frame_parity(mm, gpu, orc, tex, W, H):
+    """Every pixel of the 1080p workload frame and of the 4K frame against the oracle, in both format
+    sets (which give the same image), with the oracle's exact sample counts."""
     cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H))
     prm = mm.march_params("scenario")
-    ns = upload(mm, tex, "fast")
-    a = gpu_march(mm, ns, cam, sun, sky, W, H, params=prm)
-    b = gpu_march(mm, ns, cam, sun, sky, W, H, params=prm)
-    assert np.array_equal(a, b)  # deterministic
+    a, ca = gpu_march(mm, upload(mm, tex, "fast"), cam, sun, sky, W, H, params=prm, counts=True)
+    b, cb = gpu_march(mm, upload(mm, tex, "ref"), cam, sun, sky, W, H, params=prm, counts=True)
+    assert np.array_equal(a, b) and (ca == cb).all()
+    assert np.array_equal(a, gpu_march(mm, upload(mm, tex, "fast"), cam, sun, sky, W, H, params=prm))  # deterministic
     # alpha = max(skyAmbient, sunDisk) * (1 - accum) (comp:346, 496): up to 1 + O(1e-4) where
     # dot(sunDir, rayDirection) rounds above 1 and is raised to the 768th power (comp:330-334)
     assert np.isfinite(a).all() and (a[..., 3] >= 0).all() and (a[..., 3] <= 1.001).all()
-    rng = np.random.default_rng(1)
-    xy = np.stack([rng.integers(0, W, 4096), rng.integers(0, H, 4096)], 1).astype(np.int32)
-    r = orc.march_pixels(cam, sun, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]), prm, W, H,
-                         xy)
-    check_parity(a[xy[:, 1], xy[:, 0]], r, "1080p-sampled")
+    r, rc = orc.march(cam, sun, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]), prm, W, H,
+                      counts=True)
+    check_parity(a, r, f"{W}x{H} full frame")
+    assert int(ca[0]) == int(rc[..., 0].sum()) and int(ca[1]) == int(rc[..., 1].sum())
 
 
 def test_errors_with_real_buffers(mm, gpu, tex):

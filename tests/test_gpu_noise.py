@@ -64,8 +64,8 @@ def test_generators_fast_formats_bit_exact(mm, gpu, ref):
     ns = _gen(mm, "fast")
     assert _bytes(ns.weather) == F.pack_weather(ref["weather"], F.FMT_RG8, F.LAYOUT_APRON)
     assert _bytes(ns.curl) == F.pack_curl(ref["curl"][0], F.LAYOUT_APRON)
-    assert _bytes(ns.low_res) == F.pack_low(ref["low"], F.FMT_R8E16, F.LAYOUT_APRON)
-    assert _bytes(ns.hi_res) == F.pack_hi(ref["hi"], F.FMT_E16, F.LAYOUT_APRON)
+    assert _bytes(ns.low_res) == F.pack_low(ref["low"], F.FMT_RGBA8, F.LAYOUT_APRON)
+    assert _bytes(ns.hi_res) == F.pack_hi(ref["hi"], F.FMT_RGBA8, F.LAYOUT_APRON)
 
 
 def test_generators_precision_formats(mm, gpu, ref):
@@ -90,7 +90,7 @@ def test_low_res_ragged_dims_bit_exact(mm, gpu, orc, dim):
     import torch
 
     want = orc.low_res(dim, 11)
-    for fset, fmt, layout in (("ref", F.FMT_RGBA8, F.LAYOUT_LINEAR), ("fast", F.FMT_R8E16, F.LAYOUT_APRON)):
+    for fset, fmt, layout in (("ref", F.FMT_RGBA8, F.LAYOUT_LINEAR), ("fast", F.FMT_RGBA8, F.LAYOUT_APRON)):
         ns = mm.NoiseSet(fset, low_dim=dim)
         ns.generate(mm.noise_params(seed=11, what=mm.GEN_LOW_RES), mm.current_stream())
         torch.cuda.synchronize()
@@ -116,8 +116,8 @@ def test_pack_texture_matches_numpy(mm, gpu, ref):
 
     import torch
 
-    for kind, arr, fmt, d in [(0, ref["weather"], mm.FMT_RG8, 1), (2, ref["low"], mm.FMT_R8E16, 128),
-                              (3, ref["hi"], mm.FMT_E16, 32)]:
+    for kind, arr, fmt, d in [(0, ref["weather"], mm.FMT_RG8, 1), (2, ref["low"], mm.FMT_RGBA8, 128),
+                              (3, ref["hi"], mm.FMT_RGBA8, 32)]:
         n = arr.shape[1]
         src = mm.device_texture(n, n, d, mm.FMT_RGBA8, mm.LAYOUT_LINEAR)
         src.tensor.copy_(torch.from_numpy(arr.reshape(-1).view(np.uint8)))

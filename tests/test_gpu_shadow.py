@@ -34,14 +34,9 @@ def test_shadow_matches_oracle(mm, gpu, orc, tex, fset, sun_deg, time):
     print(fset, sun_deg, "nonzero", float((ref > 0).mean()), "max diff", float(d.max()), flush=True)
     assert (ref > 0).mean() > 0.01            # the case exercises clouds
     print("exact:", float(np.mean(d == 0)), "frac <= 1e-6:", float(np.mean(d <= 1e-6)), flush=True)
-    if fset == "ref":
-        # same formats, same operations in the same order, correctly rounded sqrt/div, one shared
-        # pow formula: equal on every point
-        assert np.array_equal(got, ref)
-    else:
-        # the packed set filters the erosion sum 5g + 2b + a: last-bit differences, which
-        # remapClamped(density, erosion, 1) amplifies by 1 / (1 - erosion) where erosion is near 1
-        assert np.mean(d <= 1e-4) >= 0.99 and d.max() <= 5e-3
+    # same UNORM8 channels in both sets, same operations in the same order, correctly rounded
+    # sqrt/div, one shared pow formula: equal on every point
+    assert np.array_equal(got, ref)
 
 
 def test_shadow_empty_and_errors(mm, gpu, tex):

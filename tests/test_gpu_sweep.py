@@ -1,6 +1,7 @@
 """The plain FP32 march over the scene sweep of the 1080p workload (coverage 0.3 / 0.5 / 0.8, sun
 5 / 25 / 60 deg, wind time 0 and 600 s, five seeded weather maps), at a small resolution, against the
-oracle on the same textures, in the reference and the packed format sets."""
+oracle on the same textures, in the reference and the packed format sets, with the oracle's exact
+sample counts."""
 import itertools
 
 import numpy as np
@@ -22,12 +23,14 @@ def shared(orc):
 
 @pytest.mark.parametrize("cov,sun,t,seed", CASES)
 def test_scene_sweep_parity(mm, gpu, orc, shared, cov, sun, t, seed):
-    W, H = 160, 90
+    W, H = 320, 180
     tex = dict(shared, weather=orc.weather(512, seed, cov))
     cam, sunu, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H, sun_elevation_deg=sun, time=t))
     prm = mm.march_params("scenario")
-    r = orc.march(cam, sunu, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]), prm, W, H)
+    r, rc = orc.march(cam, sunu, sky, orc.textures(tex["weather"], tex["curl"], tex["low"], tex["hi"]), prm, W, H,
+                      counts=True)
     for fset in ("ref", "fast"):
-        g = gpu_march(mm, upload(mm, tex, fset), cam, sunu, sky, W, H, params=prm)
+        g, c = gpu_march(mm, upload(mm, tex, fset), cam, sunu, sky, W, H, params=prm, counts=True)
         check_parity(g, r, f"cov {cov} sun {sun} t {t} seed {seed} {fset}")
+        assert int(c[0]) == int(rc[..., 0].sum()) and int(c[1]) == int(rc[..., 1].sum())
     assert np.isfinite(r).all()

@@ -182,3 +182,27 @@ def test_ray_sphere_fp32_cancellation_at_earth_radius(orc):
         assert abs(t64 - exact) < 1e-6 * max(1.0, exact)
         worst32 = max(worst32, abs(t32 - exact))
     assert worst32 > 0.05  # metres: the FP32 formula is visibly off at Earth radius
+
+
+def test_pow_rt_against_the_library_pow(orc):
+    """The coverage exponent's pow (comp:206-208) is evaluated by a written-out exp2(y log2 x) that the
+    kernel repeats operation for operation. Against float64 pow: within (1 + |y log2 x|) float ulps
+    (GLSL allows 3 + 2 |y log2 x| for exp2 alone), exact for pow(x, 1) and for powers of two whose
+    result is a power of two, and 0 for x <= 0."""
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(1e-6, 0.9, 4000), 10 ** rng.uniform(-30, 30, 4000)]).astype(np.float32)
+    y = np.concatenate([rng.uniform(0.8, 1.0, 4000), rng.uniform(-3, 3, 4000)]).astype(np.float32)
+    z = np.abs(y.astype(np.float64) * np.log2(x.astype(np.float64)))
+    keep = z < 120  # results inside the float range
+    x, y, z = x[keep], y[keep], z[keep]
+    got = np.array([orc.L.orc_pow_rt(float(a), float(b)) for a, b in zip(x, y)], np.float64)
+    ref = np.power(x.astype(np.float64), y.astype(np.float64))
+    ulps = np.abs(got - ref) / ref / 2.0 ** -23
+    assert (ulps <= 1.0 + z).all(), float((ulps / (1.0 + z)).max())
+    for v in (1e-7, 0.3, 0.9, 123.0):
+        assert orc.L.orc_pow_rt(v, 1.0) == np.float32(v)
+    for k in range(-20, 21):
+        for yy in (0.5, 0.25, 2.0, -1.0):
+            if (k * yy) == int(k * yy):
+                assert orc.L.orc_pow_rt(2.0 ** k, yy) == 2.0 ** (k * yy)
+    assert orc.L.orc_pow_rt(0.0, 0.9) == 0.0 and orc.L.orc_pow_rt(-0.5, 0.9) == 0.0
