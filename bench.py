@@ -18,13 +18,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "project-marshmallow_amd")
 METRIC = "ms/frame @1920x1080 + density samples/s; surrogate vs plain at ≤1/255 err"
 HBM_PEAK_GBS = 8000.0
+# Bytes per texel of the weather map, low-res volume, curl map and hi-res volume in each format set.
+TEXEL_BYTES = {"fast": (2, 4, 4, 4), "ref": (4, 4, 4, 4), "prec": (2, 4, 8, 8)}
 # Algorithmic texel-footprint bytes (4 weather texels + 8 low-res texels per density sample,
 # 4 curl texels + 8 hi-res texels per detail sample), per format set, plus 16 B of output per pixel.
-BYTES_PER_SAMPLE = {
-    "fast": {"density": 4 * 2 + 8 * 4, "detail": 4 * 4 + 8 * 2},
-    "ref": {"density": 4 * 4 + 8 * 4, "detail": 4 * 4 + 8 * 4},
-    "prec": {"density": 4 * 2 + 8 * 4, "detail": 4 * 8 + 8 * 8},
-}
+BYTES_PER_SAMPLE = {k: {"density": 4 * w + 8 * l, "detail": 4 * c + 8 * h} for k, (w, l, c, h) in TEXEL_BYTES.items()}
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a SIMD for one quad-cycle (4 clocks)
+SIMDS = 1024
 
 
 def load_package():
@@ -49,6 +49,22 @@ def traffic_from_profiles(fset, width, height):
             continue
         if d.get("format_set") == fset and d.get("width") == width and d.get("height") == height:
             return d.get("hbm_bytes_per_launch")
+    return None
+
+
+def valu_from_profiles(fset, width, height):
+    """VALU position of the march from a committed rocprofv3 --pmc summary of this workload, if present:
+    VALU instructions per launch, the issue fraction SQ_INSTS_VALU / (SIMDs x GRBM_GUI_ACTIVE / 4)
+    (above 1 where the SIMDs dual-issue), the share of VALU quad-cycles that dual-issued, and the lane
+    utilisation SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)."""
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*valu*.json"))):
+        try:
+            with open(path) as f:
+                d = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if d.get("format_set") == fset and d.get("width") == width and d.get("height") == height:
+            return d.get("valu")
     return None
 
 
@@ -308,10 +324,10 @@ def main():
 
     # instrumented run (outside the timed region): sample counts of this frame
     ns.generate(nparams, stream)
-    counts = torch.zeros(4, dtype=torch.int64, device="cuda")
+    counts = torch.zeros(6, dtype=torch.int64, device="cuda")
     mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream, counts=counts)
     torch.cuda.synchronize()
-    n_density, n_detail, n_jobs, _ = (int(v) for v in counts.cpu())
+    n_density, n_detail, n_jobs, _, n_weather, n_low = (int(v) for v in counts.cpu())
 
     for _ in range(args.warmup):
         ns.generate(nparams, stream)
@@ -342,11 +358,19 @@ def main():
     b = BYTES_PER_SAMPLE[args.fset]
     alg_bytes = n_density * b["density"] + n_detail * b["detail"] + W * H * 16
     achieved = alg_bytes / (march_ms * 1e-3) / 1e9
+    # the same accounting over the samples that read texels: a density sample outside the layer
+    # heights reads nothing, one on a zero layer profile reads only the weather map
+    tw, tl, _, _ = TEXEL_BYTES[args.fset]
+    gathered = n_weather * 4 * tw + n_low * 8 * tl + n_detail * b["detail"] + W * H * 16
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(args.fset, W, H),
                 "kernel": march_kernel_name(args), "kernel_ms": round(march_ms, 4),
                 "algorithmic_bytes_per_launch": alg_bytes,
-                "bytes_per_density_sample": b["density"], "bytes_per_detail_sample": b["detail"]}
+                "bytes_per_density_sample": b["density"], "bytes_per_detail_sample": b["detail"],
+                "gathered": {"bytes_per_launch": gathered, "achieved": round(gathered / (march_ms * 1e-3) / 1e9, 1),
+                             "frac": round(gathered / (march_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                             "weather_reads": n_weather, "low_res_reads": n_low},
+                "valu": valu_from_profiles(args.fset, W, H)}
     rec = {
         "metric": METRIC,
         "value": round(n_density / (ms_step * 1e-3), 1),

@@ -257,6 +257,15 @@ int mm_march_clouds_counted(const mm_camera_ubo* camera, const mm_sun_ubo* sun, 
                             int32_t phase, const mm_march_params* params,
                             unsigned long long* counts, void* stream);
 
+/* mm_march_clouds_counted with six counters: counts[0..3] as above, counts[4] = density samples that
+ * read the weather map, counts[5] = density samples that read the low-res volume (the others end
+ * before their texel reads: outside the layer heights, comp:193-204, or on a zero layer profile). */
+int mm_march_clouds_counted_ex(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                               const mm_noise_set* textures, const mm_texture* night_sky,
+                               float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
+                               int32_t phase, const mm_march_params* params,
+                               unsigned long long* counts, void* stream);
+
 /* The march with each pixel's two transmittance forms written beside its colour, for error
  * reporting against an FP64 evaluation: state_out (device, 2 floats per pixel, indexed like
  * out_rgba32f: (y * row_pitch_px + x) * 2) receives the opacity accumulator after the horizon fade

@@ -145,6 +145,9 @@ _SIGS = {
     "mm_march_clouds_counted": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
                                                ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                ctypes.c_int32, _P(MarchParams), ctypes.c_void_p, ctypes.c_void_p]),
+    "mm_march_clouds_counted_ex": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
+                                                  ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                  ctypes.c_int32, _P(MarchParams), ctypes.c_void_p, ctypes.c_void_p]),
     "mm_march_clouds_state": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
                                              ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                              ctypes.c_int32, _P(MarchParams), ctypes.c_void_p, ctypes.c_void_p]),
@@ -319,12 +322,19 @@ def current_stream() -> int:
 def march_clouds(cam: CameraUBO, sun: SunUBO, sky: SkyUBO, noise: NoiseSet, out, width: int, height: int,
                  phase: int = PHASE_ALL, params: MarchParams | None = None, night: DeviceTexture | None = None,
                  pitch: int | None = None, stream: int | None = None, counts=None) -> None:
-    """One cloud dispatch into ``out`` (a float32 device tensor of >= height*pitch*4 elements)."""
+    """One cloud dispatch into ``out`` (a float32 device tensor of >= height*pitch*4 elements). ``counts``
+    (int64 device tensor of 4 or 6 elements) accumulates the work counters of mm_march_clouds_counted
+    (4) or mm_march_clouds_counted_ex (6: also the weather-map and low-res reads)."""
     pitch = width if pitch is None else pitch
     stream = current_stream() if stream is None else stream
     pp = ctypes.byref(params) if params is not None else None
     nt = ctypes.byref(night.desc) if night is not None else None
-    if counts is None:
+    if counts is not None and counts.numel() >= 6:
+        code = lib().mm_march_clouds_counted_ex(ctypes.byref(cam), ctypes.byref(sun), ctypes.byref(sky),
+                                                ctypes.byref(noise.c), nt, out.data_ptr(), width, height, pitch,
+                                                phase, pp, counts.data_ptr(), stream)
+        check(code, "mm_march_clouds_counted_ex")
+    elif counts is None:
         code = lib().mm_march_clouds(ctypes.byref(cam), ctypes.byref(sun), ctypes.byref(sky), ctypes.byref(noise.c), nt,
                                      out.data_ptr(), width, height, pitch, phase, pp, stream)
         check(code, "mm_march_clouds")

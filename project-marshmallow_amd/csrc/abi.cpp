@@ -497,7 +497,7 @@ static void set_lanes(mm::MarchConsts& c, int lpr) {
 static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
                       const mm_noise_set* tex, const mm_texture* night_sky, float* out, int32_t width, int32_t height,
                       int32_t pitch, int32_t phase, const mm_march_params* params, unsigned long long* counts,
-                      const CollectArgs* collect, void* stream, float* state = nullptr) {
+                      const CollectArgs* collect, void* stream, float* state = nullptr, int count_slots = 4) {
     mm::MarchConsts c;
     mm::MarchTextures t;
     int fset = 0;
@@ -505,6 +505,7 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
     const int e = build_march(camera, sun, sky, tex, night_sky, out, width, height, pitch, phase, params, collect, c,
                               t, fset, empty);
     if (e || empty) return e;
+    c.countSlots = count_slots;
     if (state) {
         // the instrumented launch: the lock-step kernel (same image as every other kernel shape)
         // writes each pixel's two accumulators beside its colour
@@ -531,6 +532,15 @@ int mm_march_clouds_counted(const mm_camera_ubo* camera, const mm_sun_ubo* sun, 
     if (!counts) return MM_E_NULL;
     return march_impl(camera, sun, sky, textures, night_sky, out_rgba32f, width, height, row_pitch_px, phase, params,
                       counts, nullptr, stream);
+}
+
+int mm_march_clouds_counted_ex(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                               const mm_noise_set* textures, const mm_texture* night_sky, float* out_rgba32f,
+                               int32_t width, int32_t height, int32_t row_pitch_px, int32_t phase,
+                               const mm_march_params* params, unsigned long long* counts, void* stream) {
+    if (!counts) return MM_E_NULL;
+    return march_impl(camera, sun, sky, textures, night_sky, out_rgba32f, width, height, row_pitch_px, phase, params,
+                      counts, nullptr, stream, nullptr, 6);
 }
 
 int mm_march_clouds_state(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,

@@ -28,6 +28,13 @@ using mm::MarchTextures;
 using mm::TexView;
 using mm::ViewDesc;
 
+// Work counters of an instrumented launch, per lane: density samples (cloudTest evaluations), those
+// that read the weather map, and those that read the low-res volume, in three 21-bit fields of one
+// 64-bit value (a lane takes far fewer than 2^21 samples: at most 16384 loop trips plus its light
+// cones).
+typedef unsigned long long Tally;
+constexpr Tally kTallyWeather = 1ull << 21, kTallyLow = 1ull << 42, kTallyMask = (1ull << 21) - 1;
+
 struct V3 {
     float x, y, z;
 };
@@ -47,8 +54,11 @@ __device__ __forceinline__ float sqrt_(float x) { return sqrt_rn(x); }
 // normalize with a correctly rounded 1 / sqrt: getProjectedShellPoint scales the unit vector by the
 // shell radius (6.4e6 m in the Earth-radius scenario), so one ulp of the reciprocal moves the
 // projected point by ~0.5 m = 2e-4 of the shell height and flips density thresholds downstream.
-__device__ __forceinline__ V3 normalize3(V3 a) { return a * __fdiv_rn(1.0f, sqrt_rn(dot3(a, a))); }
-__device__ __forceinline__ float length3(V3 a) { return sqrt_rn(dot3(a, a)); }
+// Every normalised or measured vector here is a ray direction or a difference of positions at metre
+// scale (squared lengths 0 or far above 2^-96, lengths within [2^-125, 2^125]), so the unscaled forms
+// of the correctly rounded square root and reciprocal apply.
+__device__ __forceinline__ V3 normalize3(V3 a) { return a * rcp_rn(sqrt_rn_normal(dot3(a, a))); }
+__device__ __forceinline__ float length3(V3 a) { return sqrt_rn_normal(dot3(a, a)); }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 // Filter lerps: one fused multiply-add, a + t (b - a) rounded once. With 8-bit sub-texel weights every
@@ -385,12 +395,13 @@ __device__ __forceinline__ V3 wind_offset(const MarchConsts& c, float h) {
 
 template <int FSET, bool COUNT>
 __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTextures& tx, V3 pos, float h, V3 center,
-                                            int& nd) {
+                                            Tally& nd) {
     if (COUNT) nd++;
     // every layer profile is 0 at h = 0 and below 1e-7 for h >= 0.9 (the cumulus top; stratocumulus
     // and stratus end at 0.7 and 0.3), for any cloud type in [0, 1]: the density test below would
     // return 0 there whatever the textures hold, so the weather and low-res reads are skipped
     if (!(h > 0.0f && h < 0.9f)) return 0.0f;
+    if (COUNT) nd += kTallyWeather;
     const int q8 = c.subtexelBits;
     V3 proj = proj_shell(c, pos, center);
     Weather w = read_weather<FSET>(tx.weather, c.weatherScale * (proj.x - c.camPos[0]),
@@ -399,10 +410,16 @@ __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTex
     // density = layer * remapClamped(R, ...) with the remap in [0, 1]: a zero layer density gives 0
     // whatever the low-res texel, so its gather is skipped (comp:240-243 evaluate to the same 0)
     if (layer == 0.0f) return 0.0f;
+    if (COUNT) nd += kTallyLow;
     LowG<FSET> lg = gather_low<FSET>(tx.low, pos * c.lowScale, q8);
     float shape = low_shape<FSET>(lg, tx.low);
     float density = layer * clampf(0.0f + (((shape - 0.3f) * (1.0f / (1.0f - 0.3f))) * 1.0f), 0.0f, 1.0f);
     if (density < 0.0001f) return 0.0f;
+    // remapClamped(density, erosion, 1, 0, 1) with erosion in [0, 1] is exactly 1 for density > 1:
+    // (density - erosion) and (1 - erosion) round monotonically, so their quotient is >= 1 (or +inf
+    // for erosion = 1) and the clamp returns 1. The layer profile reaches 5, so dense cloud interiors
+    // skip the erosion channels, the coverage pow and both remap quotients.
+    if (density > 1.0f) return 1.0f;
     // heightBiasCoverage(relativeHeight, min(0.85, cloudInfo.r)) (comp:206-208, 245)
     float e = clampf(remapk(fminf(0.85f, w.coverage), 0.7f, 1.0f / (0.8f - 0.7f), 1.0f, 0.8f - 1.0f), 0.8f, 1.0f);
     float coverage = pow_rt(h, e);
@@ -636,7 +653,7 @@ __device__ __forceinline__ float4 compose(const MarchConsts& c, const RaySetup& 
 // Density along the sun: the six-sample light cone (comp:440-453).
 template <int FSET, bool COUNT>
 __device__ __forceinline__ float cone_dal(const MarchConsts& c, const MarchTextures& tx, V3 currentPos, float stepSize,
-                                          int& nd, int& nh) {
+                                          Tally& nd, int& nh) {
     V3 center = ld3(c.camPos);
     center.y = c.earthCenter[1];
     float dal = 0.0f;
@@ -666,7 +683,7 @@ __device__ __forceinline__ float beers_of(float dal, float beerBlend) {
 // position within the hi-res tile as sin/cos pairs.
 template <int FSET, bool COUNT>
 __device__ __forceinline__ void cone_features(const MarchConsts& c, const MarchTextures& tx, V3 pos, float stepSize,
-                                              float rh, float lo, float dens, float* f, int& nd) {
+                                              float rh, float lo, float dens, float* f, Tally& nd) {
     V3 center = ld3(c.camPos);
     center.y = c.earthCenter[1];
     const V3 p = pos + wind_offset(c, rh);
@@ -719,7 +736,8 @@ __device__ __forceinline__ void march_init(const MarchConsts& c, const RaySetup&
 struct TripSample {
     V3 pos;
     float rHeight, lo, dens;
-    int nd, nh;  // density / detail samples taken (counted once the trip commits)
+    Tally nd;
+    int nh;  // density / detail samples taken (counted once the trip commits)
 };
 
 template <int FSET, bool COUNT>
@@ -804,7 +822,7 @@ __device__ __forceinline__ bool trip_commit(const MarchConsts& c, const RaySetup
 template <int FSET, bool COUNT>
 __device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchTextures& tx, const RaySetup& r,
                                            MarchState& m, V3& jobPos, float& jobStep, float& jobX0, float& jobW,
-                                           float& jobRh, float& jobLo, float& jobDens, int& nd, int& nh) {
+                                           float& jobRh, float& jobLo, float& jobDens, Tally& nd, int& nh) {
     if (!trip_begin(c, m)) return false;
     const TripSample s = trip_eval<FSET, COUNT>(c, tx, r, m.noHits, m.stepSize, m.t);
     const float step = m.stepSize;
@@ -845,6 +863,16 @@ __device__ __forceinline__ void tile_pixel(const MarchConsts& c, int gi, int gj,
     }
 }
 
+// Adds a lane's density tally to counts[0] and, for a six-slot counter array, its weather and
+// low-res reads to counts[4] and counts[5].
+__device__ __forceinline__ void add_tally(const MarchConsts& c, unsigned long long* counts, Tally nd) {
+    atomicAdd(&counts[0], nd & kTallyMask);
+    if (c.countSlots >= 6) {
+        atomicAdd(&counts[4], (nd >> 21) & kTallyMask);
+        atomicAdd(&counts[5], (nd >> 42) & kTallyMask);
+    }
+}
+
 // ------------------------------------------------------------------ lock-step kernel
 // One wave per 64-pixel tile; every lane evaluates its light cones in place (the shader's structure).
 template <int FSET, bool COUNT>
@@ -858,7 +886,8 @@ __global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const Ma
     if (gi >= c.gridW || gj >= c.gridH) return;
     int px, py;
     tile_pixel(c, gi, gj, px, py);
-    int nd = 0, nh = 0, nj = 0, nc = 0;  // density, detail samples; cone jobs, jobs on the plain cone
+    Tally nd = 0;
+    int nh = 0, nj = 0, nc = 0;  // density, detail samples; cone jobs, jobs on the plain cone
     const RaySetup r = ray_setup(c, tx, px, py);
     float4* dst = reinterpret_cast<float4*>(out) + ((size_t)py * c.pitch + px);
     float2* st = c.stateOut ? reinterpret_cast<float2*>(c.stateOut) + ((size_t)py * c.pitch + px) : nullptr;
@@ -882,7 +911,7 @@ __global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const Ma
     *dst = compose(c, r, m.accum, transmittance);
     if (st) *st = make_float2(final_accum(r, m.accum), transmittance);
     if (COUNT) {
-        atomicAdd(&counts[0], (unsigned long long)nd);
+        add_tally(c, counts, nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
         atomicAdd(&counts[2], (unsigned long long)nj);
         atomicAdd(&counts[3], (unsigned long long)nc);
@@ -927,7 +956,7 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
 // shuffle rather than stored in the queue.
 template <int FSET, bool COUNT, int MODE, class Q>
 __device__ __forceinline__ void run_batch(const MarchConsts& c, const MarchTextures& tx, Q& q, const int nb,
-                                          const float myBlend, int& nd, int& nh, int& nj, int& nc) {
+                                          const float myBlend, Tally& nd, int& nh, int& nj, int& nc) {
     const int lane = threadIdx.x;
     const float blend = __shfl(myBlend, q.owner[lane]);  // every lane takes part; slots >= nb are unused
     if (COUNT) nj += nb;
@@ -1026,7 +1055,8 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
     const bool valid = gi < c.gridW && gj < c.gridH;
     int px = 0, py = 0;
     if (valid) tile_pixel(c, gi, gj, px, py);
-    int nd = 0, nh = 0, nj = 0, nc = 0;  // density, detail samples; cone jobs, jobs on the plain cone
+    Tally nd = 0;
+    int nh = 0, nj = 0, nc = 0;  // density, detail samples; cone jobs, jobs on the plain cone
     RaySetup r;
     MarchState m;
     m.active = false;
@@ -1068,7 +1098,7 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
     }
     if (valid && r.march) reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
     if (COUNT) {
-        atomicAdd(&counts[0], (unsigned long long)nd);
+        add_tally(c, counts, nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
         if (lane == 0) {
             atomicAdd(&counts[2], (unsigned long long)nj);
@@ -1107,7 +1137,8 @@ __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchT
     const bool valid = gi < c.gridW && gj < c.gridH;
     int px = 0, py = 0;
     if (valid) tile_pixel(c, gi, gj, px, py);
-    int nd = 0, nh = 0, nj = 0, nc = 0;
+    Tally nd = 0;
+    int nh = 0, nj = 0, nc = 0;
     RaySetup r;
     MarchState m;
     m.active = false;
@@ -1174,7 +1205,7 @@ __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchT
     if (valid && r.march && k == 0)
         reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
     if (COUNT) {
-        atomicAdd(&counts[0], (unsigned long long)nd);
+        add_tally(c, counts, nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
         if (lane == 0) {
             atomicAdd(&counts[2], (unsigned long long)nj);

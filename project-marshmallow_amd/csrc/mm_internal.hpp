@@ -48,6 +48,7 @@ struct MarchConsts {
     const float* surrParams;      // FP32 parameters of the surrogate MLP
     const float* surrStats;       // feature mean[16], inverse std[16]
     const uint4* surrFrags;       // BF16 MFMA weight fragments
+    int countSlots;               // 4, or 6: counts[4], counts[5] also receive the weather / low-res reads
     float* stateOut;              // lock-step kernel only, or null: per pixel (final opacity accumulator,
                                   // light-energy accumulator), 2 floats at the image's pixel index
     float* collectBuf;            // [collectCap][17]: 16 features + density along the light

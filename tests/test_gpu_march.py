@@ -324,3 +324,25 @@ def test_cone_batching_is_bit_identical(mm, gpu, orc, tex, fset, preset):
     b, cb = gpu_march(mm, ns, cam, sun, sky, W, H, params=prm_lock, counts=True)
     assert np.array_equal(a, b)
     assert (ca == cb).all()
+
+
+def test_six_counter_launch(mm, gpu, tex):
+    """mm_march_clouds_counted_ex: the first four counters equal mm_march_clouds_counted's, and the
+    texture-read counters nest: every detail sample follows a density sample that read the low-res
+    volume, which read the weather map first."""
+    import torch
+
+    W, H = 200, 120
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H))
+    ns = upload(mm, tex, "fast")
+    for lanes in (1, 2, 8):
+        prm = mm.march_params("scenario", lanes_per_ray=lanes)
+        out = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+        c4 = torch.zeros(4, dtype=torch.int64, device="cuda")
+        c6 = torch.zeros(6, dtype=torch.int64, device="cuda")
+        mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, counts=c4)
+        mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, counts=c6)
+        torch.cuda.synchronize()
+        a, b = [int(v) for v in c4.cpu()], [int(v) for v in c6.cpu()]
+        assert b[:4] == a, (lanes, a, b)
+        assert b[0] >= b[4] >= b[5] >= b[1] > 0, (lanes, b)
