@@ -476,7 +476,10 @@ static int lanes_for(const mm::MarchConsts& c, const mm_march_params* params, in
     int lpr = params ? params->lanes_per_ray : 0;
     if (lpr == 0)
         lpr = (phase >= 0 || phase == MM_PHASE_FROM_UBO) ? kDefaultPhaseLanesPerRay : kDefaultFrameLanesPerRay;
-    if (c.coneMode != MM_CONE_PLAIN || (c.flags & MM_MARCH_NO_BATCH)) lpr = 1;
+    if (c.flags & MM_MARCH_NO_BATCH) return 1;
+    // the surrogate-lit march runs with one or two lanes per ray; the sample recorder with one
+    if (c.coneMode == MM_CONE_SURROGATE) return lpr >= 2 ? 2 : 1;
+    if (c.coneMode != MM_CONE_PLAIN) return 1;
     return lpr;
 }
 

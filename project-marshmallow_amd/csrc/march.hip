@@ -393,9 +393,11 @@ __device__ __forceinline__ V3 wind_offset(const MarchConsts& c, float h) {
     return (c.windStrength * w) * (c.time + h * 200.0f);
 }
 
+// wout (optional) receives the weather coverage and type the test read (left untouched when the
+// height skips the read).
 template <int FSET, bool COUNT>
 __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTextures& tx, V3 pos, float h, V3 center,
-                                            Tally& nd) {
+                                            Tally& nd, Weather* wout = nullptr) {
     if (COUNT) nd++;
     // every layer profile is 0 at h = 0 and below 1e-7 for h >= 0.9 (the cumulus top; stratocumulus
     // and stratus end at 0.7 and 0.3), for any cloud type in [0, 1]: the density test below would
@@ -406,6 +408,7 @@ __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTex
     V3 proj = proj_shell(c, pos, center);
     Weather w = read_weather<FSET>(tx.weather, c.weatherScale * (proj.x - c.camPos[0]),
                                    c.weatherScale * (proj.z - c.camPos[2]), q8);
+    if (wout) *wout = w;
     float layer = layer_density(h, w.type);
     // density = layer * remapClamped(R, ...) with the remap in [0, 1]: a zero layer density gives 0
     // whatever the low-res texel, so its gather is skipped (comp:240-243 evaluate to the same 0)
@@ -681,16 +684,16 @@ __device__ __forceinline__ float beers_of(float dal, float beerBlend) {
 // eroded density), weather coverage/type at the hit, sun elevation, three density probes along the
 // sun direction (the cone's axial offsets 0.6, 2.5 and 6 step lengths x 3), step scale, and the
 // position within the hi-res tile as sin/cos pairs.
+// The coverage and type are those the hit's own density test read (the weather map at the
+// projection of the wind-offset hit position), carried in the job queue.
 template <int FSET, bool COUNT>
 __device__ __forceinline__ void cone_features(const MarchConsts& c, const MarchTextures& tx, V3 pos, float stepSize,
-                                              float rh, float lo, float dens, float* f, Tally& nd) {
+                                              float rh, float lo, float dens, float cov, float type, float* f,
+                                              Tally& nd) {
     V3 center = ld3(c.camPos);
     center.y = c.earthCenter[1];
     const V3 p = pos + wind_offset(c, rh);
-    const V3 proj = proj_shell(c, p, center);
-    const Weather w = read_weather<FSET>(tx.weather, c.weatherScale * (proj.x - c.camPos[0]),
-                                         c.weatherScale * (proj.z - c.camPos[2]), c.subtexelBits);
-    f[0] = rh; f[1] = lo; f[2] = dens; f[3] = w.coverage; f[4] = w.type; f[5] = c.sunDir[1];
+    f[0] = rh; f[1] = lo; f[2] = dens; f[3] = cov; f[4] = type; f[5] = c.sunDir[1];
 #pragma unroll 1
     for (int i = 0; i < 3; i++) {
         V3 lsPos = pos + ((3.0f * stepSize) * ld3(c.probeOffsets[i]));
@@ -736,15 +739,17 @@ __device__ __forceinline__ void march_init(const MarchConsts& c, const RaySetup&
 struct TripSample {
     V3 pos;
     float rHeight, lo, dens;
+    Weather w;  // the weather the density test read (kept for the light-cone surrogate's features)
     Tally nd;
     int nh;  // density / detail samples taken (counted once the trip commits)
 };
 
-template <int FSET, bool COUNT>
+template <int FSET, bool COUNT, bool KEEPW = false>
 __device__ __forceinline__ TripSample trip_eval(const MarchConsts& c, const MarchTextures& tx, const RaySetup& r,
                                                 bool noHits, float stepSize, float t) {
     TripSample s;
     s.nd = s.nh = 0;
+    s.w.coverage = s.w.type = 0.0f;
     const V3 camPos = ld3(c.camPos);
     V3 center = camPos;
     center.y = c.earthCenter[1];
@@ -752,7 +757,7 @@ __device__ __forceinline__ TripSample trip_eval(const MarchConsts& c, const Marc
     const V3 proj = proj_shell(c, s.pos, center);
     s.rHeight = rel_height(c, s.pos, proj);
     const V3 wo = wind_offset(c, s.rHeight);
-    s.lo = cloud_test<FSET, COUNT>(c, tx, s.pos + wo, s.rHeight, center, s.nd);
+    s.lo = cloud_test<FSET, COUNT>(c, tx, s.pos + wo, s.rHeight, center, s.nd, KEEPW ? &s.w : nullptr);
     s.dens = s.lo;
     if (s.lo > 0.0f && !noHits) s.dens = cloud_hires<FSET, COUNT>(c, tx, s.pos + wo, stepSize, s.lo, s.rHeight, s.nh);
     return s;
@@ -819,12 +824,13 @@ __device__ __forceinline__ bool trip_commit(const MarchConsts& c, const RaySetup
 }
 
 // One trip of the reference's for-loop: its samples, then its state update.
-template <int FSET, bool COUNT>
+template <int FSET, bool COUNT, bool KEEPW = false>
 __device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchTextures& tx, const RaySetup& r,
                                            MarchState& m, V3& jobPos, float& jobStep, float& jobX0, float& jobW,
-                                           float& jobRh, float& jobLo, float& jobDens, Tally& nd, int& nh) {
+                                           float& jobRh, float& jobLo, float& jobDens, Weather& jobWeather, Tally& nd,
+                                           int& nh) {
     if (!trip_begin(c, m)) return false;
-    const TripSample s = trip_eval<FSET, COUNT>(c, tx, r, m.noHits, m.stepSize, m.t);
+    const TripSample s = trip_eval<FSET, COUNT, KEEPW>(c, tx, r, m.noHits, m.stepSize, m.t);
     const float step = m.stepSize;
     const bool job = trip_commit(c, r, m, s.lo, s.dens, jobW);
     if (job) {
@@ -834,6 +840,7 @@ __device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchText
         jobRh = s.rHeight;
         jobLo = s.lo;
         jobDens = s.dens;
+        jobWeather = s.w;
     }
     nd += s.nd;
     nh += s.nh;
@@ -902,7 +909,8 @@ __global__ __launch_bounds__(64) void march_kernel(const MarchConsts c, const Ma
     while (m.active) {
         V3 jp;
         float js, jx0, jw, jrh, jlo, jd;
-        if (march_trip<FSET, COUNT>(c, tx, r, m, jp, js, jx0, jw, jrh, jlo, jd, nd, nh)) {
+        Weather jwe;
+        if (march_trip<FSET, COUNT>(c, tx, r, m, jp, js, jx0, jw, jrh, jlo, jd, jwe, nd, nh)) {
             float dal = cone_dal<FSET, COUNT>(c, tx, jp, js, nd, nh);
             transmittance = mixf(transmittance, jx0 * beers_of(dal, r.beerBlend), jw);
             if (COUNT) { nj++; nc++; }
@@ -938,7 +946,7 @@ struct ConeQueuePlain {
     int owner[kQueueCap];
 };
 struct ConeQueueFull : ConeQueuePlain {
-    float rh[kQueueCap], lo[kQueueCap], dens[kQueueCap];
+    float rh[kQueueCap], lo[kQueueCap], dens[kQueueCap], cov[kQueueCap], type[kQueueCap];
     int fallback[64];
     float dal[64];
 };
@@ -979,7 +987,9 @@ __device__ __forceinline__ void run_batch(const MarchConsts& c, const MarchTextu
             const uint32_t hsh = mix32(__float_as_uint(pos.x) ^ mix32(__float_as_uint(pos.y) ^
                                        mix32(__float_as_uint(pos.z) ^ c.collectSeed)));
             keep = (hsh % c.collectKeep) == 0u;
-            if (keep) cone_features<FSET, COUNT>(c, tx, pos, q.step[lane], q.rh[lane], q.lo[lane], q.dens[lane], f, nd);
+            if (keep)
+                cone_features<FSET, COUNT>(c, tx, pos, q.step[lane], q.rh[lane], q.lo[lane], q.dens[lane], q.cov[lane],
+                                           q.type[lane], f, nd);
         }
         const unsigned long long km = __ballot(keep);
         if (km) {
@@ -1002,7 +1012,7 @@ __device__ __forceinline__ void run_batch(const MarchConsts& c, const MarchTextu
         for (int k = 0; k < surr::kIn; k++) f[k] = 0.0f;
         if (lane < nb)
             cone_features<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], q.rh[lane],
-                                       q.lo[lane], q.dens[lane], f, nd);
+                                       q.lo[lane], q.dens[lane], q.cov[lane], q.type[lane], f, nd);
         for (int k = 0; k < surr::kIn; k++) f[k] = (f[k] - c.surrStats[k]) * c.surrStats[16 + k];
         const surr::FwdOut o = surr::mlp_forward64(c.surrFrags, c.surrParams, f);
         // the job's share of the pixel: its cone result enters T with weight (1 - accum) * inScatter * HG
@@ -1037,6 +1047,7 @@ __device__ __forceinline__ void queue_shift(Q& q, const int nb, const int rem) {
         q.x0[lane] = q.x0[s]; q.w[lane] = q.w[s]; q.owner[lane] = q.owner[s];
         if constexpr (sizeof(Q) == sizeof(ConeQueueFull)) {
             q.rh[lane] = q.rh[s]; q.lo[lane] = q.lo[s]; q.dens[lane] = q.dens[s];
+            q.cov[lane] = q.cov[s]; q.type[lane] = q.type[s];
         }
     }
     __syncthreads();
@@ -1073,8 +1084,11 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
         while (qn < 64 && __any(m.active)) {
             V3 jp;
             float js = 0.f, jx0 = 0.f, jw = 0.f, jrh = 0.f, jlo = 0.f, jd = 0.f;
+            Weather jwe;
+            jwe.coverage = jwe.type = 0.0f;
             bool job = false;
-            if (m.active) job = march_trip<FSET, COUNT>(c, tx, r, m, jp, js, jx0, jw, jrh, jlo, jd, nd, nh);
+            if (m.active)
+                job = march_trip<FSET, COUNT, MODE != CONE_PLAIN>(c, tx, r, m, jp, js, jx0, jw, jrh, jlo, jd, jwe, nd, nh);
             const unsigned long long mask = __ballot(job);
             if (job) {
                 const int slot = qn + __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
@@ -1083,6 +1097,7 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
                 q.x0[slot] = jx0; q.w[slot] = jw; q.owner[slot] = lane;
                 if constexpr (MODE != CONE_PLAIN) {
                     q.rh[slot] = jrh; q.lo[slot] = jlo; q.dens[slot] = jd;
+                    q.cov[slot] = jwe.coverage; q.type[slot] = jwe.type;
                 }
             }
             qn += __popcll(mask);
@@ -1125,12 +1140,12 @@ __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, 
 // committed trips are exactly the loop's, so the image is the one-lane-per-ray image. It shortens
 // a ray's critical path by up to LPR, which is what bounds a single 1/16 phase (two waves per SIMD,
 // each as long as its longest ray). Light-cone jobs go to the wave's queue like the batched kernel's.
-template <int FSET, bool COUNT, int LPR>
+template <int FSET, bool COUNT, int LPR, int MODE = CONE_PLAIN>
 __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchTextures& tx, const int tile,
                                               float* __restrict__ out, unsigned long long* __restrict__ counts) {
     static_assert(LPR == 2 || LPR == 4 || LPR == 8, "lanes per ray");
     constexpr int TH = 64 / LPR / 8;  // tile: 8 x TH rays
-    __shared__ ConeQueuePlain q;
+    __shared__ typename ConeQueueOf<MODE>::type q;
     const int lane = threadIdx.x, ray = lane / LPR, k = lane % LPR, first = lane - k;
     const int gi = (tile % c.tilesX) * 8 + (ray & 7);
     const int gj = tile_row(c, tile / c.tilesX) * TH + (ray >> 3);
@@ -1160,7 +1175,7 @@ __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchT
             s.pos = mk(0.f, 0.f, 0.f);
             s.rHeight = s.lo = s.dens = 0.0f;
             s.nd = s.nh = 0;
-            if (m.active) s = trip_eval<FSET, COUNT>(c, tx, r, noHits0, step0, tk);
+            if (m.active) s = trip_eval<FSET, COUNT, MODE != CONE_PLAIN>(c, tx, r, noHits0, step0, tk);
             float lo[LPR], dens[LPR];
 #pragma unroll
             for (int j = 0; j < LPR; j++) {
@@ -1190,13 +1205,17 @@ __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchT
                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
                 q.x[slot] = s.pos.x; q.y[slot] = s.pos.y; q.z[slot] = s.pos.z; q.step[slot] = step0;
                 q.x0[slot] = inscatter_hg(s.lo, s.rHeight, r.HG); q.w[slot] = jw; q.owner[slot] = lane;
+                if constexpr (MODE != CONE_PLAIN) {
+                    q.rh[slot] = s.rHeight; q.lo[slot] = s.lo; q.dens[slot] = s.dens;
+                    q.cov[slot] = s.w.coverage; q.type[slot] = s.w.type;
+                }
             }
             qn += __popcll(mask);
         }
         if (qn == 0) break;
         __syncthreads();
         const int nb = qn < 64 ? qn : 64;
-        run_batch<FSET, COUNT, CONE_PLAIN>(c, tx, q, nb, valid ? r.beerBlend : 0.0f, nd, nh, nj, nc);
+        run_batch<FSET, COUNT, MODE>(c, tx, q, nb, valid ? r.beerBlend : 0.0f, nd, nh, nj, nc);
         for (int j = 0; j < nb; j++)
             if (q.owner[j] / LPR == ray) transmittance = mixf(transmittance, q.res[j], q.w[j]);
         queue_shift(q, nb, qn - nb);
@@ -1236,6 +1255,16 @@ void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* 
     const int tile = blockIdx.x;
     if (tile >= c.tiles) return;
     march_tile<FSET, COUNT, CONE_SURROGATE>(c, tx, tile, out, counts);
+}
+
+// The surrogate-lit march with two lanes per ray (the plain march's default shape for full frames).
+template <int FSET, bool COUNT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
+void march_rp_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
+                               unsigned long long* __restrict__ counts) {
+    const int tile = blockIdx.x;
+    if (tile >= c.tiles) return;
+    march_rp_tile<FSET, COUNT, 2, CONE_SURROGATE>(c, tx, tile, out, counts);
 }
 
 // ------------------------------------------------------------------ cloud shadows (model.frag)
@@ -1385,7 +1414,9 @@ int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned 
     else if (c.coneMode == CONE_PLAIN)
         hipLaunchKernelGGL((march_batched_kernel<FSET, COUNT, CONE_PLAIN>), grid, block, 0, s, c, t, out, counts);
     else if constexpr (FSET != mm::FSET_PREC) {
-        if (c.coneMode == CONE_COLLECT)
+        if (c.coneMode == CONE_SURROGATE && c.lanesPerRay == 2)
+            hipLaunchKernelGGL((march_rp_surrogate_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
+        else if (c.coneMode == CONE_COLLECT)
             hipLaunchKernelGGL((march_batched_kernel<FSET, COUNT, CONE_COLLECT>), grid, block, 0, s, c, t, out, counts);
         else
             hipLaunchKernelGGL((march_surrogate_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);

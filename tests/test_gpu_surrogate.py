@@ -192,3 +192,29 @@ def test_trained_surrogate_gain(mm, gpu):
             if r["mean_err_255"] <= 1.0 and r["replaced_frac"] >= 0.5:
                 ok.append(thr)
         assert ok, "no threshold meets 1/255 mean error with half of the cones replaced"
+
+
+def test_surrogate_lanes_per_ray_do_not_change_the_image(mm, gpu):
+    """The surrogate-lit march with one lane per ray (cone-batched kernel) and with two (ray-parallel
+    kernel, the default for a full frame): the same jobs reach the network and the fallback, in
+    another queue order, so the image and the counters are the same bit for bit; likewise one phase."""
+    import torch
+
+    W, H = 200, 120
+    ns, cam, sun, sky = _scene(mm, W, H, sun=35.0, coverage=0.7, seed=3)
+    m = mm.Surrogate(seed=4)
+    for phase in (mm.PHASE_ALL, 6):
+        outs, cnts = [], []
+        for lanes in (1, 2, 0):
+            ps = mm.march_params("scenario", cone_mode=mm.CONE_SURROGATE, surrogate_threshold=0.01, surrogate=m.ptr,
+                                 lanes_per_ray=lanes)
+            o = torch.full((W * H * 4,), -1.0, dtype=torch.float32, device="cuda")
+            c = torch.zeros(4, dtype=torch.int64, device="cuda")
+            mm.march_clouds(cam, sun, sky, ns, o, W, H, phase=phase, params=ps, counts=c)
+            outs.append(o)
+            cnts.append(c)
+        torch.cuda.synchronize()
+        print("phase", phase, "jobs", int(cnts[0][2]), "on the plain cone", int(cnts[0][3]), flush=True)
+        assert int(cnts[0][2]) > 0
+        for o, c in zip(outs[1:], cnts[1:]):
+            assert torch.equal(o, outs[0]) and torch.equal(c, cnts[0]), phase
