@@ -419,7 +419,8 @@ void mm_view_plan_destroy(mm_view_plan* plan);
 
 /* ---------------------------------------------------------------- learned light-cone surrogate
  * No counterpart in the reference: an optional replacement of the six-sample light cone
- * (comp:440-453) by a 16-64-64-64-2 SiLU MLP evaluated with BF16 MFMA inside the march. For each
+ * (comp:440-453) by a 16-64-64-64-2 MLP (hard-SiLU hidden units, z * clamp(z / 6 + 1/2, 0, 1))
+ * evaluated with BF16 MFMA inside the march. For each
  * cone job it predicts the density along the light D and an error bound e on exp(-D); the job's
  * result enters the pixel's transmittance with weight (1 - accum) * inScatter * HG (comp:464), and
  * jobs whose weighted bound e * (1 - accum) * |inScatter * HG| is >= params.surrogate_threshold

@@ -3,7 +3,7 @@ the smoke check and the CPU leg of the benchmark, never by the product path).
 
 The surrogate has no counterpart in the reference: it stands in for the six-sample light cone of
 compute-clouds.comp:440-453 (density along the sun, D, then Beer's law). This module restates the
-network the HIP kernels implement -- a 16-64-64-64-2 MLP with SiLU, outputs (o0, o1) with
+network the HIP kernels implement -- a 16-64-64-64-2 MLP with hard-SiLU units, outputs (o0, o1) with
 o0 ~ -D and softplus(o1) = e an error bound in transmittance units -- its training loss and the
 exact gradient of that loss, in float64.
 
@@ -54,13 +54,13 @@ def _q(x, on):
     return bf16(x) if on else x
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
+def hswish(z):
+    """Hidden-unit activation: hard SiLU (hard-swish), z * clamp(z / 6 + 1/2, 0, 1)."""
+    return z * np.clip(z / 6.0 + 0.5, 0.0, 1.0)
 
 
-def silu_grad(z):
-    s = 1.0 / (1.0 + np.exp(-z))
-    return s * (1.0 + z * (1.0 - s))
+def hswish_grad(z):
+    return np.where(z <= -3.0, 0.0, np.where(z >= 3.0, 1.0, z / 3.0 + 0.5))
 
 
 def softplus(x):
@@ -83,7 +83,7 @@ def forward(flat: np.ndarray, xn: np.ndarray, bf16_ops: bool = False):
     for l in (1, 2, 3):
         z = a @ _q(p[f"W{l}"], bf16_ops) + p[f"b{l}"]
         zs.append(z)
-        a = _q(silu(z), bf16_ops)
+        a = _q(hswish(z), bf16_ops)
     o = a @ _q(p["W4"], bf16_ops) + p["b4"]
     return o, zs
 
@@ -127,14 +127,14 @@ def gradient(flat, stats, rows, bf16_ops: bool = False):
     L = float(np.sum(r * r + (e - ar) ** 2))
     dout = np.stack([2.0 * r * pe, 2.0 * (e - ar) * sig], 1) / n
     q = lambda v: _q(v, bf16_ops)
-    acts = [q(xn)] + [q(silu(z)) for z in zs]  # inputs of layers 1..4
+    acts = [q(xn)] + [q(hswish(z)) for z in zs]  # inputs of layers 1..4
     g = {}
     dz = q(dout)
     g["W4"] = acts[3].T @ dz
     g["b4"] = dz.sum(0)
     da = dz @ q(p["W4"]).T
     for l in (3, 2, 1):
-        dz = q(da * silu_grad(zs[l - 1]))
+        dz = q(da * hswish_grad(zs[l - 1]))
         g[f"W{l}"] = acts[l - 1].T @ dz
         g[f"b{l}"] = dz.sum(0)
         if l > 1:

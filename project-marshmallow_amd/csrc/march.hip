@@ -683,7 +683,9 @@ __device__ __forceinline__ float beers_of(float dal, float beerBlend) {
 // Inputs of the light-cone surrogate for one job: hit state (relative height, low-res density,
 // eroded density), weather coverage/type at the hit, sun elevation, three density probes along the
 // sun direction (the cone's axial offsets 0.6, 2.5 and 6 step lengths x 3), step scale, and the
-// position within the hi-res tile as sin/cos pairs.
+// position within the hi-res tile as sin/cos pairs. (Cheaper probes -- the low-res base shape alone,
+// scaled by the layer profile -- cost a third as much and made the trained network's error on the
+// 1080p frame with every cone replaced 3.5 times larger, 1.68 against 0.48 / 255.)
 // The coverage and type are those the hit's own density test read (the weather map at the
 // projection of the wind-offset hit position), carried in the job queue.
 template <int FSET, bool COUNT>
@@ -1014,7 +1016,7 @@ __device__ __forceinline__ void run_batch(const MarchConsts& c, const MarchTextu
             cone_features<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], q.rh[lane],
                                        q.lo[lane], q.dens[lane], q.cov[lane], q.type[lane], f, nd);
         for (int k = 0; k < surr::kIn; k++) f[k] = (f[k] - c.surrStats[k]) * c.surrStats[16 + k];
-        const surr::FwdOut o = surr::mlp_forward64(c.surrFrags, c.surrParams, f);
+        const surr::FwdOut o = surr::mlp_forward64(c.surrFrags, c.surrParams, f, nb);
         // the job's share of the pixel: its cone result enters T with weight (1 - accum) * inScatter * HG
         const float bound = lane < nb ? surr::softplus(o.o1) * (q.w[lane] * fabsf(q.x0[lane])) : 0.0f;
         const bool fb = lane < nb && !(bound < c.surrThreshold);
@@ -1242,12 +1244,13 @@ __global__ __launch_bounds__(64) void march_rp_kernel(const MarchConsts c, const
 }
 
 #ifndef MM_SURROGATE_WAVES_PER_EU
-#define MM_SURROGATE_WAVES_PER_EU 6
+#define MM_SURROGATE_WAVES_PER_EU 5
 #endif
 // The surrogate variant bounds its register allocation: the MLP's peak (accumulators, operands)
-// otherwise sets the occupancy of the whole march. At 6 waves per SIMD the MLP spills some registers
-// to scratch and the whole launch is still the fastest (1080p, every cone replaced: 3 / 4 / 5 / 6
-// waves 2.43 / 2.24 / 2.11 / 2.10 ms).
+// otherwise sets the occupancy of the whole march. Its job queue (7 KB per wave with the features'
+// inputs) already holds a CU to 22 waves, 5 per SIMD, so the cap is 5: at 6 the MLP spills to
+// scratch for no extra wave (1080p, every cone replaced, two lanes per ray: 4 / 5 / 6 waves
+// 2.27 / 2.15 / 2.22 ms).
 template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
 void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,

@@ -120,11 +120,6 @@ __global__ void pack_kernel(const float* __restrict__ p, uint4* __restrict__ fra
     frags[t] = __builtin_bit_cast(uint4, b);
 }
 
-__device__ __forceinline__ float silu_grad(float z) {
-    const float s = 1.0f / (1.0f + __expf(-z));
-    return s * (1.0f + z * (1.0f - s));
-}
-
 // One Adam step's gradients over a batch: block = one wave = 64 random samples (two 32-sample
 // tiles). Forward in registers (pre-activations parked in LDS), backward dA = W dZ with the same
 // fragment layout, weight gradients dW^T = dZ^T a^T from LDS-transposed tiles, accumulated into
@@ -196,12 +191,12 @@ __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ sam
             dz4[1] = L.dout[32 * tile + c][1] * invB;
         }
         // dW4^T[u][k] = sum_s dZ4[u][s] a3[k][s], db4: accumulate directly (2 outputs x 64 units)
-        // a3 = silu(z3) of this lane's sample c for units crow(r, h) + 32 mt
+        // a3 = hswish(z3) of this lane's sample c for units crow(r, h) + 32 mt
         {
             // write a3^T and dz4^T tiles to LDS
             for (int mt = 0; mt < 2; mt++)
                 for (int r = 0; r < 16; r++)
-                    L.act[mt * 32 + crow(r, h)][c] = (__bf16)silu(L.z[tile][2][mt][r][lane]);
+                    L.act[mt * 32 + crow(r, h)][c] = (__bf16)hswish(L.z[tile][2][mt][r][lane]);
             if (h == 0) {
                 L.dz[0][c] = (__bf16)dz4[0];
                 L.dz[1][c] = (__bf16)dz4[1];
@@ -230,14 +225,14 @@ __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ sam
             }
             __syncthreads();
         }
-        // dA3 = W4 dZ4 -> dZ3 = dA3 * silu'(z3)
+        // dA3 = W4 dZ4 -> dZ3 = dA3 * hswish'(z3)
         f32x16 dz[2];
         {
             const bf16x8 b = pack8(dz4);
             for (int mt = 0; mt < 2; mt++) {
                 f32x16 z = {};
                 z = mfma(load_frag(frags, kFragB4 + mt, lane), b, z);
-                for (int r = 0; r < 16; r++) z[r] *= silu_grad(L.z[tile][2][mt][r][lane]);
+                for (int r = 0; r < 16; r++) z[r] *= hswish_grad(L.z[tile][2][mt][r][lane]);
                 dz[mt] = z;
             }
         }
@@ -252,7 +247,7 @@ __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ sam
             if (layer > 0) {
                 for (int mt = 0; mt < 2; mt++)
                     for (int r = 0; r < 16; r++)
-                        L.act[mt * 32 + crow(r, h)][c] = (__bf16)silu(L.z[tile][layer - 1][mt][r][lane]);
+                        L.act[mt * 32 + crow(r, h)][c] = (__bf16)hswish(L.z[tile][layer - 1][mt][r][lane]);
             } else {
                 for (int k = 0; k < kIn; k++)
                     if (h == 0) L.act[k][c] = (__bf16)L.x[32 * tile + c][k];
@@ -284,7 +279,7 @@ __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ sam
             }
             __syncthreads();
             if (layer == 0) break;
-            // dA_{l-1} = W_l dZ_l (B operand = dZ registers, k permuted), times silu'(z_{l-1})
+            // dA_{l-1} = W_l dZ_l (B operand = dZ registers, k permuted), times hswish'(z_{l-1})
             bf16x8 b[4];
             for (int mt = 0; mt < 2; mt++) {
                 float v[16];
@@ -295,7 +290,7 @@ __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ sam
             for (int mt = 0; mt < 2; mt++) {
                 f32x16 z = {};
                 for (int kt = 0; kt < 4; kt++) z = mfma(load_frag(frags, bfrag[li] + mt * 4 + kt, lane), b[kt], z);
-                for (int r = 0; r < 16; r++) z[r] *= silu_grad(L.z[tile][layer - 1][mt][r][lane]);
+                for (int r = 0; r < 16; r++) z[r] *= hswish_grad(L.z[tile][layer - 1][mt][r][lane]);
                 dz[mt] = z;
             }
         }
@@ -332,7 +327,8 @@ __global__ __launch_bounds__(64) void predict_kernel(const float* __restrict__ x
     const long i = (long)blockIdx.x * 64 + lane;
     float f[kIn];
     for (int k = 0; k < kIn; k++) f[k] = i < n ? (x[i * stride + k] - stats[k]) * stats[16 + k] : 0.0f;
-    FwdOut o = mlp_forward64(frags, params, f);
+    const long left = (long)n - (long)blockIdx.x * 64;
+    FwdOut o = mlp_forward64(frags, params, f, left < 64 ? (int)left : 64);
     if (i < n) {
         out[i * 2 + 0] = o.o0;
         out[i * 2 + 1] = softplus(o.o1);

@@ -1,4 +1,4 @@
-// Device side of the learned light-cone surrogate: a 16-64-64-64-2 MLP (SiLU) evaluated with
+// Device side of the learned light-cone surrogate: a 16-64-64-64-2 MLP evaluated with
 // gfx950 BF16 MFMA (v_mfma_f32_32x32x16_bf16), FP32 accumulation, shared by the training kernels
 // and by the march's light-cone batches.
 //
@@ -37,7 +37,16 @@ __host__ __device__ inline int kperm(int kt, int h, int j) { return (kt / 2) * 3
 // Row of accumulator element r in lane half h.
 __host__ __device__ inline int crow(int r, int h) { return 8 * (r / 4) + 4 * h + (r % 4); }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// Hidden-unit activation: hard SiLU (hard-swish, MobileNetV3), z * clamp(z / 6 + 1/2, 0, 1), the
+// piecewise-quadratic form of SiLU z * sigmoid(z): three full-rate VALU operations instead of an
+// exponential and a reciprocal, which at 192 activations per job cost as much as a plain light cone.
+__device__ __forceinline__ float hswish(float z) {
+    return z * __builtin_amdgcn_fmed3f(__builtin_fmaf(z, 1.0f / 6.0f, 0.5f), 0.0f, 1.0f);
+}
+// d hswish / dz: 0 below -3, 1 above 3, z / 3 + 1/2 between
+__device__ __forceinline__ float hswish_grad(float z) {
+    return z <= -3.0f ? 0.0f : (z >= 3.0f ? 1.0f : __builtin_fmaf(z, 1.0f / 3.0f, 0.5f));
+}
 
 __device__ __forceinline__ bf16x8 load_frag(const uint4* frags, int frag, int lane) {
     uint4 u = frags[frag * 64 + lane];
@@ -75,13 +84,13 @@ __device__ __forceinline__ FwdOut mlp_forward_tile(const uint4* __restrict__ fra
     const int bias[3] = {oB1, oB2, oB3};
     const int ffrag[2] = {kFragF2, kFragF3};
     for (int layer = 0; layer < 3; layer++) {
-        // bias + SiLU, repack as the next layer's B operand (8 consecutive accumulators = one k-step)
+        // bias + activation, repack as the next layer's B operand (8 consecutive accumulators = one k-step)
         for (int mt = 0; mt < 2; mt++) {
             float v[16];
             for (int r = 0; r < 16; r++) {
                 float zz = acc[mt][r] + params[bias[layer] + mt * 32 + crow(r, h)];
                 if (zsave) zsave[(layer * 2 + mt) * 16 + r] = zz;
-                v[r] = silu(zz);
+                v[r] = hswish(zz);
             }
             b[mt * 2 + 0] = pack8(v);
             b[mt * 2 + 1] = pack8(v + 8);
@@ -112,7 +121,6 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
     return (T*)(((uint64_t)hi << 32) | (uint64_t)lo);
 }
 
-__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 // Bias of rows crow(r, h) of m-tile `off` as an accumulator initialiser.
 __device__ __forceinline__ f32x16 bias16(const float* __restrict__ p, int off, int h) {
@@ -122,7 +130,8 @@ __device__ __forceinline__ f32x16 bias16(const float* __restrict__ p, int off, i
 }
 
 // Inference forward of one 32-sample tile with a small register footprint: one m-tile at a time,
-// biases folded into the accumulator initialiser, weight fragments loaded just before their MFMA.
+// biases folded into the accumulator initialiser. The four weight fragments of an m-tile are loaded
+// together before its MFMA chain, so the chain waits for one load latency rather than four.
 __device__ __forceinline__ FwdOut mlp_infer_tile(const uint4* __restrict__ frags, const float* __restrict__ params,
                                                  bf16x8 xb) {
     const int lane = threadIdx.x & 63, h = lane >> 5;
@@ -131,7 +140,7 @@ __device__ __forceinline__ FwdOut mlp_infer_tile(const uint4* __restrict__ frags
         f32x16 z = bias16(params, oB1 + mt * 32, h);
         z = mfma(load_frag(frags, kFragF1 + mt, lane), xb, z);
         float v[16];
-        for (int r = 0; r < 16; r++) v[r] = silu_fast(z[r]);
+        for (int r = 0; r < 16; r++) v[r] = hswish(z[r]);
         b[mt * 2] = pack8(v);
         b[mt * 2 + 1] = pack8(v + 8);
     }
@@ -139,33 +148,43 @@ __device__ __forceinline__ FwdOut mlp_infer_tile(const uint4* __restrict__ frags
     const int ffrag[2] = {kFragF2, kFragF3};
 #pragma unroll 1
     for (int layer = 0; layer < 2; layer++) {
+        // the two m-tiles' MFMA chains interleaved: each dependent MFMA has the other chain's in front
+        bf16x8 w0[4], w1[4];
+        for (int kt = 0; kt < 4; kt++) {
+            w0[kt] = load_frag(frags, ffrag[layer] + kt, lane);
+            w1[kt] = load_frag(frags, ffrag[layer] + 4 + kt, lane);
+        }
+        f32x16 zz[2] = {bias16(params, bias[layer], h), bias16(params, bias[layer] + 32, h)};
+        for (int kt = 0; kt < 4; kt++) {
+            zz[0] = mfma(w0[kt], b[kt], zz[0]);
+            zz[1] = mfma(w1[kt], b[kt], zz[1]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
         for (int mt = 0; mt < 2; mt++) {
-            f32x16 z = bias16(params, bias[layer] + mt * 32, h);
-            for (int kt = 0; kt < 4; kt++) {
-                z = mfma(load_frag(frags, ffrag[layer] + mt * 4 + kt, lane), b[kt], z);
-                __builtin_amdgcn_sched_barrier(0);
-            }
+            const f32x16 z = zz[mt];
             float v[16];
-            for (int r = 0; r < 16; r++) v[r] = silu_fast(z[r]);
+            for (int r = 0; r < 16; r++) v[r] = hswish(z[r]);
             nb[mt * 2] = pack8(v);
             nb[mt * 2 + 1] = pack8(v + 8);
         }
         for (int k = 0; k < 4; k++) b[k] = nb[k];
     }
+    bf16x8 w[4];
+    for (int kt = 0; kt < 4; kt++) w[kt] = load_frag(frags, kFragF4 + kt, lane);
     f32x16 o = {};
-    for (int kt = 0; kt < 4; kt++) {
-        o = mfma(load_frag(frags, kFragF4 + kt, lane), b[kt], o);
-        __builtin_amdgcn_sched_barrier(0);
-    }
+    for (int kt = 0; kt < 4; kt++) o = mfma(w[kt], b[kt], o);
+    __builtin_amdgcn_sched_barrier(0);
     FwdOut r;
     r.o0 = o[0] + params[oB4 + 0];  // row crow(0, 0) = 0 in lanes of half 0
     r.o1 = o[1] + params[oB4 + 1];  // row 1
     return r;
 }
 
-// Inference for 64 samples (two tiles), one sample per lane. `x` are this lane's normalised
-// features. Every lane of the wave must call it (MFMA is wave-wide). Returns this lane's outputs.
-__device__ __forceinline__ FwdOut mlp_forward64(const uint4* frags, const float* params, const float* x) {
+// Inference for up to 64 samples (two tiles), one sample per lane. `x` are this lane's normalised
+// features; `n` (wave-uniform) is the number of samples in use, lanes 0..n-1: with n <= 32 the second
+// tile (samples 32..63) is not evaluated and lanes 32..63 return zeros. Every lane of the wave must
+// call it (MFMA is wave-wide). Returns this lane's outputs.
+__device__ __forceinline__ FwdOut mlp_forward64(const uint4* frags, const float* params, const float* x, int n = 64) {
     const int lane = threadIdx.x & 63, h = lane >> 5;
     // opaque to loop-invariant code motion: keeps the weight loads from being hoisted out of an
     // enclosing loop (where they would hold registers for the whole loop)
@@ -183,7 +202,9 @@ __device__ __forceinline__ FwdOut mlp_forward64(const uint4* frags, const float*
     const bf16x8 b1 = h ? hi : xlo;  // tile 1 (samples 32..63)
     FwdOut t0 = mlp_infer_tile(frags, params, b0);
     __builtin_amdgcn_sched_barrier(0);
-    FwdOut t1 = mlp_infer_tile(frags, params, b1);
+    FwdOut t1;
+    t1.o0 = t1.o1 = 0.0f;
+    if (n > 32) t1 = mlp_infer_tile(frags, params, b1);  // wave-uniform: a half-empty batch skips its second tile
     // tile 0 results sit in lane c (= sample c); tile 1 results for sample 32 + c sit in lane c
     FwdOut r;
     const float s0 = __shfl_xor(t1.o0, 32), s1 = __shfl_xor(t1.o1, 32);

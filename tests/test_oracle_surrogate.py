@@ -84,3 +84,14 @@ def test_bf16_gradient_close_to_float64():
     g64, _ = S.gradient(flat, stats, rows)
     g16, _ = S.gradient(flat, stats, rows, bf16_ops=True)
     assert np.max(np.abs(g64 - g16)) <= 0.05 * np.max(np.abs(g64))
+
+
+def test_hard_silu_activation():
+    """Hidden units: hard SiLU z * clamp(z / 6 + 1/2, 0, 1) -- zero below -3, the identity above 3,
+    z^2 / 6 + z / 2 between -- and its derivative against central differences away from the kinks."""
+    z = np.array([-10.0, -3.0, -1.5, 0.0, 1.5, 3.0, 10.0])
+    assert np.allclose(S.hswish(z), [0.0, 0.0, -0.375, 0.0, 1.125, 3.0, 10.0])
+    zz = np.linspace(-6, 6, 1201)
+    zz = zz[np.abs(np.abs(zz) - 3.0) > 1e-3]
+    fd = (S.hswish(zz + 1e-6) - S.hswish(zz - 1e-6)) / 2e-6
+    assert np.allclose(S.hswish_grad(zz), fd, atol=1e-6)
