@@ -825,6 +825,7 @@ int mm_post_process(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const fl
 
 struct mm_view_plan {
     int n_views = 0, total_tiles = 0, fset = 0, mode = 0, lanes = 1;
+    bool float_weights = false;       // subtexel_bits 0: the float-sampler kernels
     mm::ViewDesc* d_views = nullptr;  // device: ViewDesc[n_views]
     mm::MarchTextures shared;
     mm_noise_set shared_set;          // the shared textures as given (curl, low_res, hi_res)
@@ -898,6 +899,7 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
     p->lanes = lanes;
     p->fset = fset;
     p->mode = params->cone_mode;
+    p->float_weights = params->subtexel_bits == 0;
     p->shared = tx;
     p->shared_set = *shared;
     p->tile_base.resize((size_t)n_views);
@@ -933,6 +935,7 @@ int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_view
         params = &ref;
     }
     if (params->cone_mode != plan->mode) return MM_E_PARAM;
+    if ((params->subtexel_bits == 0) != plan->float_weights) return MM_E_PARAM;  // a different kernel
     // the shared volumes stay those of the plan; views bring new uniforms, weather maps and outputs
     std::vector<mm::ViewDesc> h;
     mm::MarchTextures tx;
@@ -965,8 +968,8 @@ int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* s
     if (!plan) return MM_E_NULL;
     if (plan->total_tiles == 0) return MM_OK;
     reset_error_state();
-    return mm::launch_views(plan->fset, plan->mode, plan->lanes, plan->d_views, plan->n_views, plan->shared,
-                            plan->total_tiles, counts, stream);
+    return mm::launch_views(plan->fset, plan->mode, plan->lanes, plan->float_weights, plan->d_views, plan->n_views,
+                            plan->shared, plan->total_tiles, counts, stream);
 }
 
 void mm_view_plan_destroy(mm_view_plan* plan) {

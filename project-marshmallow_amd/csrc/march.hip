@@ -23,6 +23,13 @@
 
 namespace {
 
+// The FSET template argument of the kernels: a format set of mm.h (FSET_REF, FSET_FAST, FSET_PREC), or'ed
+// with kFloatWeights for the float sampler (subtexel_bits 0). Without the flag the sub-texel weights
+// are rounded to 1/256 like the hardware sampler's, and 8-bit channels are filtered in integers.
+constexpr int kFloatWeights = 8;
+template <int F> constexpr int kSet = F & 7;
+template <int F> constexpr int kQ8 = (F & kFloatWeights) ? 0 : 1;
+
 using mm::MarchConsts;
 using mm::MarchTextures;
 using mm::TexView;
@@ -131,17 +138,21 @@ __device__ __forceinline__ float remapClamped01(float v, float a) {
 // u * n is exact (n a power of two), so fma(u, n, -0.5) rounds like u * n - 0.5. With the 1/256 rounding,
 // k = floor(x * 256 + 0.5) (x * 256 exact) gives floor(k / 256) = k >> 8 and the fraction
 // (k & 255) / 256 exactly as the float steps do, for texel coordinates below 2^23 in magnitude
-// (8.4 M texels; the presets stay below 1e5).
-__device__ __forceinline__ void axis_p2(float u, int n, int q8, int& i0, float& f) {
+// (8.4 M texels; the presets stay below 1e5). With the rounding, w = (256 - fk) | fk << 16 holds the
+// integer weights of the two texels (fk = k & 255) for the integer filter below.
+__device__ __forceinline__ void axis_p2(float u, int n, int q8, int& i0, float& f, uint32_t& w) {
     const float x = __builtin_fmaf(u, (float)n, -0.5f);
     if (q8) {
         const int k = (int)floorf(__builtin_fmaf(x, 256.0f, 0.5f));
         i0 = (k >> 8) & (n - 1);
-        f = (float)(k & 255) * (1.0f / 256.0f);
+        const uint32_t fk = (uint32_t)k & 255u;
+        f = (float)fk * (1.0f / 256.0f);
+        w = (256u - fk) | (fk << 16);
     } else {
         const float fl = floorf(x);
         f = x - fl;
         i0 = ((int)fl) & (n - 1);
+        w = 0u;
     }
 }
 
@@ -184,13 +195,18 @@ __device__ __forceinline__ void ld_pair(const uint8_t* p, typename TexelT<BYTES>
     }
 }
 
+// A gathered footprint: texels, float fractions and, with 1/256 weights (q8), the integer weight pairs.
 template <int BYTES> struct G2 {
     typename TexelT<BYTES>::type t[4];
     float fx, fy;
+    uint32_t wx, wy;
+    int q8;
 };
 template <int BYTES> struct G3 {
     typename TexelT<BYTES>::type t[8];
     float fx, fy, fz;
+    uint32_t wx, wy, wz;
+    int q8;
 };
 
 // Gather of the 2x2 bilinear footprint. APRON: the texture stores n+1 texels per axis with texel n
@@ -198,9 +214,10 @@ template <int BYTES> struct G3 {
 template <int BYTES, bool APRON>
 __device__ __forceinline__ G2<BYTES> gather2(const TexView& tv, float u, float v, int q8) {
     G2<BYTES> g;
+    g.q8 = q8;
     int x0, y0;
-    axis_p2(u, tv.w, q8, x0, g.fx);
-    axis_p2(v, tv.h, q8, y0, g.fy);
+    axis_p2(u, tv.w, q8, x0, g.fx, g.wx);
+    axis_p2(v, tv.h, q8, y0, g.fy, g.wy);
     int x1, y1;
     if (APRON) { x1 = x0 + 1; y1 = y0 + 1; }
     else { x1 = (x0 + 1) & (tv.w - 1); y1 = (y0 + 1) & (tv.h - 1); }
@@ -223,11 +240,12 @@ __device__ __forceinline__ G2<BYTES> gather2(const TexView& tv, float u, float v
 template <int BYTES, bool APRON>
 __device__ __forceinline__ G3<BYTES> gather3(const TexView& tv, float u, float v, float w, int q8) {
     G3<BYTES> g;
+    g.q8 = q8;
     int x0, y0, z0;
     const int n = tv.w;
-    axis_p2(u, n, q8, x0, g.fx);
-    axis_p2(v, n, q8, y0, g.fy);
-    axis_p2(w, n, q8, z0, g.fz);
+    axis_p2(u, n, q8, x0, g.fx, g.wx);
+    axis_p2(v, n, q8, y0, g.fy, g.wy);
+    axis_p2(w, n, q8, z0, g.fz, g.wz);
     int x1, y1, z1;
     if (APRON) { x1 = x0 + 1; y1 = y0 + 1; z1 = z0 + 1; }
     else { x1 = (x0 + 1) & (n - 1); y1 = (y0 + 1) & (n - 1); z1 = (z0 + 1) & (n - 1); }
@@ -271,6 +289,42 @@ __device__ __forceinline__ float filt3(const G3<BYTES>& g, F ch) {
     return lerpf(s0, s1, g.fz);
 }
 
+// Integer filtering of an 8-bit channel with 1/256 weights. With t in 0..255 and f = fk / 256, the float
+// steps above are exact through two levels -- lerp(a, b, f) = (a (256 - fk) + b fk) / 256 needs 16 bits,
+// the second level 24 -- and the third rounds once (one FMA on exact operands): filt2 = M / 2^16 and
+// filt3 = RN(N / 2^24) for the integer sums M < 2^24, N < 2^32. The same sums in integer arithmetic --
+// byte pairs by v_perm_b32, each level a v_dot2_u32_u16 against the weight pair (256 - fk, fk), the last
+// level of filt3 in 24-bit multiplies -- then one conversion (RN(N) 2^-24 = RN(N / 2^24)) give the same
+// bits in 16 instead of 23 operations per channel.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t dot2u(uint32_t a, uint32_t w, uint32_t c) {
+    return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, w), c, false);
+}
+// (byte K of a, byte K of b) as a 16-bit pair (v_perm_b32 selector 12 gives a zero byte)
+template <int K>
+__device__ __forceinline__ uint32_t pair8(uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_perm(b, a, (0x0cu << 24) | ((4u + (uint32_t)K) << 16) | (0x0cu << 8) | (uint32_t)K);
+}
+// (low half of a, low half of b) as a 16-bit pair
+__device__ __forceinline__ uint32_t pair16(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x05040100u); }
+
+template <int K, int BYTES>
+__device__ __forceinline__ uint32_t isum2(const G2<BYTES>& g) {
+    const uint32_t l0 = dot2u(pair8<K>(g.t[0], g.t[1]), g.wx, 0u);
+    const uint32_t l1 = dot2u(pair8<K>(g.t[2], g.t[3]), g.wx, 0u);
+    return dot2u(pair16(l0, l1), g.wy, 0u);
+}
+template <int K, int BYTES>
+__device__ __forceinline__ uint32_t isum3(const G3<BYTES>& g) {
+    const uint32_t a0 = dot2u(pair8<K>(g.t[0], g.t[1]), g.wx, 0u);
+    const uint32_t b0 = dot2u(pair8<K>(g.t[2], g.t[3]), g.wx, 0u);
+    const uint32_t a1 = dot2u(pair8<K>(g.t[4], g.t[5]), g.wx, 0u);
+    const uint32_t b1 = dot2u(pair8<K>(g.t[6], g.t[7]), g.wx, 0u);
+    const uint32_t s0 = dot2u(pair16(a0, b0), g.wy, 0u);
+    const uint32_t s1 = dot2u(pair16(a1, b1), g.wy, 0u);
+    return __umul24(s0, g.wz & 0xffffu) + __umul24(s1, g.wz >> 16);
+}
+
 // Channel decoders.
 template <int K> struct Byte {
     __device__ __forceinline__ float operator()(uint32_t v) const { return (float)((v >> (8 * K)) & 0xffu); }
@@ -290,6 +344,18 @@ template <int K> struct Fp8 {  // channel K of 4 x FP8 e4m3 (OCP) packed in a ui
 
 constexpr float kInv255 = 1.0f / 255.0f;
 
+// Byte channel K filtered and scaled: filt(...) * s (s * 2^-16 and s * 2^-24 are exact scalings of s)
+template <int K, int BYTES>
+__device__ __forceinline__ float chan2(const G2<BYTES>& g, float s) {
+    if (g.q8) return (float)isum2<K>(g) * (s * 0x1p-16f);
+    return filt2(g, Byte<K>()) * s;
+}
+template <int K, int BYTES>
+__device__ __forceinline__ float chan3(const G3<BYTES>& g, float s) {
+    if (g.q8) return (float)isum3<K>(g) * (s * 0x1p-24f);
+    return filt3(g, Byte<K>()) * s;
+}
+
 // ------------------------------------------------------------------ per-format texture reads
 struct Weather {
     float coverage, type;
@@ -298,25 +364,41 @@ struct Weather {
 template <int FSET>
 __device__ __forceinline__ Weather read_weather(const TexView& tv, float u, float v, int q8) {
     Weather w;
-    if constexpr (FSET == mm::FSET_REF) {
+    if constexpr (kSet<FSET> == mm::FSET_REF) {
         G2<4> g = gather2<4, false>(tv, u, v, q8);
-        w.coverage = filt2(g, Byte<0>()) * kInv255;
-        w.type = filt2(g, Byte<2>()) * kInv255;
+        if (q8) {
+            const uint32_t c = isum2<0>(g), t = isum2<2>(g);
+            w.coverage = (float)c * (kInv255 * 0x1p-16f);
+            w.type = (float)t * (kInv255 * 0x1p-16f);
+        } else {
+            w.coverage = filt2(g, Byte<0>()) * kInv255;
+            w.type = filt2(g, Byte<2>()) * kInv255;
+        }
     } else {
         G2<2> g = gather2<2, true>(tv, u, v, q8);
-        w.coverage = filt2(g, Byte<0>()) * kInv255;
-        w.type = filt2(g, Byte<1>()) * kInv255;
+        if (q8) {
+            const uint32_t c = isum2<0>(g), t = isum2<1>(g);
+            w.coverage = (float)c * (kInv255 * 0x1p-16f);
+            w.type = (float)t * (kInv255 * 0x1p-16f);
+        } else {
+            w.coverage = filt2(g, Byte<0>()) * kInv255;
+            w.type = filt2(g, Byte<1>()) * kInv255;
+        }
     }
     return w;
 }
 
 template <int FSET>
 __device__ __forceinline__ V3 read_curl(const TexView& tv, float u, float v, int q8) {
-    if constexpr (FSET == mm::FSET_PREC) {
+    if constexpr (kSet<FSET> == mm::FSET_PREC) {
         G2<8> g = gather2<8, true>(tv, u, v, q8);
         return mk(filt2(g, Half4<0>()), filt2(g, Half4<1>()), filt2(g, Half4<2>()));
     } else {
-        G2<4> g = gather2<4, FSET != mm::FSET_REF>(tv, u, v, q8);
+        G2<4> g = gather2<4, kSet<FSET> != mm::FSET_REF>(tv, u, v, q8);
+        if (q8) {
+            const uint32_t x = isum2<0>(g), y = isum2<1>(g), z = isum2<2>(g);
+            return mk((float)x * (kInv255 * 0x1p-16f), (float)y * (kInv255 * 0x1p-16f), (float)z * (kInv255 * 0x1p-16f));
+        }
         return mk(filt2(g, Byte<0>()) * kInv255, filt2(g, Byte<1>()) * kInv255, filt2(g, Byte<2>()) * kInv255);
     }
 }
@@ -324,13 +406,23 @@ __device__ __forceinline__ V3 read_curl(const TexView& tv, float u, float v, int
 // Hi-res erosion 0.625 r + 0.25 g + 0.125 b of the filtered texel (comp:223-224).
 template <int FSET>
 __device__ __forceinline__ float read_hi_erosion(const TexView& tv, V3 p, int q8) {
-    if constexpr (FSET == mm::FSET_PREC) {
+    if constexpr (kSet<FSET> == mm::FSET_PREC) {
         G3<8> g = gather3<8, true>(tv, p.x, p.y, p.z, q8);
         float r = filt3(g, Half4<0>()), gg = filt3(g, Half4<1>()), b = filt3(g, Half4<2>());
         return ((0.625f * r) + (0.25f * gg)) + (0.125f * b);
     } else {
-        G3<4> g = gather3<4, FSET != mm::FSET_REF>(tv, p.x, p.y, p.z, q8);
-        float r = filt3(g, Byte<0>()) * kInv255, gg = filt3(g, Byte<1>()) * kInv255, b = filt3(g, Byte<2>()) * kInv255;
+        G3<4> g = gather3<4, kSet<FSET> != mm::FSET_REF>(tv, p.x, p.y, p.z, q8);
+        float r, gg, b;
+        if (q8) {
+            const uint32_t nr = isum3<0>(g), ng = isum3<1>(g), nb = isum3<2>(g);
+            r = (float)nr * (kInv255 * 0x1p-24f);
+            gg = (float)ng * (kInv255 * 0x1p-24f);
+            b = (float)nb * (kInv255 * 0x1p-24f);
+        } else {
+            r = filt3(g, Byte<0>()) * kInv255;
+            gg = filt3(g, Byte<1>()) * kInv255;
+            b = filt3(g, Byte<2>()) * kInv255;
+        }
         return ((0.625f * r) + (0.25f * gg)) + (0.125f * b);
     }
 }
@@ -344,23 +436,32 @@ template <int FSET> struct LowG {
 template <int FSET>
 __device__ __forceinline__ LowG<FSET> gather_low(const TexView& tv, V3 p, int q8) {
     LowG<FSET> r;
-    r.g = gather3<4, FSET != mm::FSET_REF>(tv, p.x, p.y, p.z, q8);
+    r.g = gather3<4, kSet<FSET> != mm::FSET_REF>(tv, p.x, p.y, p.z, q8);
     return r;
 }
 template <int FSET>
 __device__ __forceinline__ float low_shape(const LowG<FSET>& l, const TexView& tv) {
-    if constexpr (FSET == mm::FSET_PREC) return filt3(l.g, Fp8<0>()) * tv.scale[0];
-    else return filt3(l.g, Byte<0>()) * kInv255;
+    if constexpr (kSet<FSET> == mm::FSET_PREC) return filt3(l.g, Fp8<0>()) * tv.scale[0];
+    else return chan3<0>(l.g, kInv255);
 }
 template <int FSET>
 __device__ __forceinline__ float low_erosion(const LowG<FSET>& l, const TexView& tv) {
-    if constexpr (FSET == mm::FSET_PREC) {
+    if constexpr (kSet<FSET> == mm::FSET_PREC) {
         float g = filt3(l.g, Fp8<1>()) * tv.scale[1], b = filt3(l.g, Fp8<2>()) * tv.scale[2],
               a = filt3(l.g, Fp8<3>()) * tv.scale[3];
         return ((0.625f * g) + (0.25f * b)) + (0.125f * a);
     } else {
-        float g = filt3(l.g, Byte<1>()) * kInv255, b = filt3(l.g, Byte<2>()) * kInv255,
-              a = filt3(l.g, Byte<3>()) * kInv255;
+        float g, b, a;
+        if (l.g.q8) {
+            const uint32_t ng = isum3<1>(l.g), nb = isum3<2>(l.g), na = isum3<3>(l.g);
+            g = (float)ng * (kInv255 * 0x1p-24f);
+            b = (float)nb * (kInv255 * 0x1p-24f);
+            a = (float)na * (kInv255 * 0x1p-24f);
+        } else {
+            g = filt3(l.g, Byte<1>()) * kInv255;
+            b = filt3(l.g, Byte<2>()) * kInv255;
+            a = filt3(l.g, Byte<3>()) * kInv255;
+        }
         return ((0.625f * g) + (0.25f * b)) + (0.125f * a);
     }
 }
@@ -404,7 +505,7 @@ __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTex
     // return 0 there whatever the textures hold, so the weather and low-res reads are skipped
     if (!(h > 0.0f && h < 0.9f)) return 0.0f;
     if (COUNT) nd += kTallyWeather;
-    const int q8 = c.subtexelBits;
+    constexpr int q8 = kQ8<FSET>;
     V3 proj = proj_shell(c, pos, center);
     Weather w = read_weather<FSET>(tx.weather, c.weatherScale * (proj.x - c.camPos[0]),
                                    c.weatherScale * (proj.z - c.camPos[2]), q8);
@@ -435,7 +536,7 @@ template <int FSET, bool COUNT>
 __device__ __forceinline__ float cloud_hires(const MarchConsts& c, const MarchTextures& tx, V3 pos, float curlStrength,
                                              float orig, float h, int& nh) {
     if (COUNT) nh++;
-    const int q8 = c.subtexelBits;
+    constexpr int q8 = kQ8<FSET>;
     V3 cv = read_curl<FSET>(tx.curl, c.curlScale * pos.x, c.curlScale * pos.z, q8);
     V3 curl = mk(2.0f * cv.x - 1.0f, 2.0f * cv.y - 1.0f, 2.0f * cv.z - 1.0f);
     pos = pos + ((1.9f * curlStrength) * curl);
@@ -1295,9 +1396,9 @@ __device__ float sh_cloud_test(const mm::ShadowConsts& c, const TexView& wt, con
                                V3 center) {
     const V3 proj = (c.projRadius * normalize3(pos - center)) + center;
     const Weather w = read_weather<FSET>(wt, c.weatherScale * (proj.x - c.camPos[0]),
-                                                c.weatherScale * (proj.z - c.camPos[2]), c.subtexelBits);
+                                                c.weatherScale * (proj.z - c.camPos[2]), kQ8<FSET>);
     const float layer = sh_layer(h, w.type);
-    const LowG<FSET> lg = gather_low<FSET>(lt, pos * c.lowScale, c.subtexelBits);
+    const LowG<FSET> lg = gather_low<FSET>(lt, pos * c.lowScale, kQ8<FSET>);
     float density = layer * sh_remap_clamped(low_shape<FSET>(lg, lt), 0.3f, 1.0f, 0.0f, 1.0f);
     if (density < 0.0001f) return 0.0f;
     const float e = clampf(sh_remap(fminf(0.85f, w.coverage), 0.7f, 0.8f, 1.0f, 0.6f), 0.6f, 1.0f);
@@ -1386,21 +1487,46 @@ void march_views_surrogate_kernel(const ViewDesc* __restrict__ views, int nViews
 
 template <int FSET, bool COUNT>
 const void* views_kernel_ptr(int mode, int lanes) {
+    if constexpr ((FSET & kFloatWeights) != 0) lanes = 1;  // one kernel shape for the float sampler
     if (mode == CONE_SURROGATE) {
-        if constexpr (FSET != mm::FSET_PREC) return (const void*)march_views_surrogate_kernel<FSET, COUNT>;
+        if constexpr (kSet<FSET> != mm::FSET_PREC) return (const void*)march_views_surrogate_kernel<FSET, COUNT>;
         return nullptr;
     }
-    switch (lanes) {
-        case 2: return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN, 2>;
-        case 4: return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN, 4>;
-        case 8: return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN, 8>;
+    if constexpr ((FSET & kFloatWeights) == 0) {
+        switch (lanes) {
+            case 2: return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN, 2>;
+            case 4: return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN, 4>;
+            case 8: return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN, 8>;
+        }
     }
     return (const void*)march_views_kernel<FSET, COUNT, CONE_PLAIN, 1>;
 }
 
 template <int FSET>
-const void* views_kernel_ptr_c(int mode, int lanes, bool count) {
+const void* views_kernel_ptr_c(int mode, int lanes, bool count, bool floatWeights) {
+    if (floatWeights)
+        return count ? views_kernel_ptr<FSET | kFloatWeights, true>(mode, lanes)
+                     : views_kernel_ptr<FSET | kFloatWeights, false>(mode, lanes);
     return count ? views_kernel_ptr<FSET, true>(mode, lanes) : views_kernel_ptr<FSET, false>(mode, lanes);
+}
+
+// The float sampler (subtexel_bits 0) is a checking mode: one kernel per cone mode, the lock-step one
+// for plain cones (every kernel shape gives the same image and counts).
+template <int FSET, bool COUNT>
+int launch_float_weights(const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts,
+                         hipStream_t s) {
+    dim3 grid((unsigned)c.tiles), block(64);
+    if (c.coneMode == CONE_PLAIN)
+        hipLaunchKernelGGL((march_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
+    else if constexpr (kSet<FSET> != mm::FSET_PREC) {
+        if (c.coneMode == CONE_COLLECT)
+            hipLaunchKernelGGL((march_batched_kernel<FSET, COUNT, CONE_COLLECT>), grid, block, 0, s, c, t, out, counts);
+        else
+            hipLaunchKernelGGL((march_surrogate_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
+    } else {
+        return MM_E_FORMAT;
+    }
+    return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 
 template <int FSET, bool COUNT>
@@ -1416,7 +1542,7 @@ int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned 
         hipLaunchKernelGGL((march_rp_kernel<FSET, COUNT, 2>), grid, block, 0, s, c, t, out, counts);
     else if (c.coneMode == CONE_PLAIN)
         hipLaunchKernelGGL((march_batched_kernel<FSET, COUNT, CONE_PLAIN>), grid, block, 0, s, c, t, out, counts);
-    else if constexpr (FSET != mm::FSET_PREC) {
+    else if constexpr (kSet<FSET> != mm::FSET_PREC) {
         if (c.coneMode == CONE_SURROGATE && c.lanesPerRay == 2)
             hipLaunchKernelGGL((march_rp_surrogate_kernel<FSET, COUNT>), grid, block, 0, s, c, t, out, counts);
         else if (c.coneMode == CONE_COLLECT)
@@ -1437,8 +1563,13 @@ int launch_shadow(int fset, const ShadowConsts& c, const TexView& weather, const
                   float* out, void* stream) {
     dim3 grid((unsigned)((c.n + 255) / 256)), block(256);
     hipStream_t s = (hipStream_t)stream;
-    if (fset == FSET_REF)
+    const bool fw = c.subtexelBits == 0;
+    if (fset == FSET_REF && fw)
+        hipLaunchKernelGGL(cloud_shadow_kernel<FSET_REF | kFloatWeights>, grid, block, 0, s, c, weather, low, pts, out);
+    else if (fset == FSET_REF)
         hipLaunchKernelGGL(cloud_shadow_kernel<FSET_REF>, grid, block, 0, s, c, weather, low, pts, out);
+    else if (fset == FSET_FAST && fw)
+        hipLaunchKernelGGL(cloud_shadow_kernel<FSET_FAST | kFloatWeights>, grid, block, 0, s, c, weather, low, pts, out);
     else if (fset == FSET_FAST)
         hipLaunchKernelGGL(cloud_shadow_kernel<FSET_FAST>, grid, block, 0, s, c, weather, low, pts, out);
     else
@@ -1446,13 +1577,13 @@ int launch_shadow(int fset, const ShadowConsts& c, const TexView& weather, const
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 
-int launch_views(int fset, int mode, int lanesPerRay, const ViewDesc* views, int nViews, const MarchTextures& t,
-                 int total, unsigned long long* counts, void* stream) {
+int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, const ViewDesc* views, int nViews,
+                 const MarchTextures& t, int total, unsigned long long* counts, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     const bool cnt = counts != nullptr;
-    const void* k = fset == FSET_REF ? views_kernel_ptr_c<FSET_REF>(mode, lanesPerRay, cnt)
-                    : fset == FSET_FAST ? views_kernel_ptr_c<FSET_FAST>(mode, lanesPerRay, cnt)
-                                        : views_kernel_ptr_c<FSET_PREC>(mode, lanesPerRay, cnt);
+    const void* k = fset == FSET_REF ? views_kernel_ptr_c<FSET_REF>(mode, lanesPerRay, cnt, floatWeights)
+                    : fset == FSET_FAST ? views_kernel_ptr_c<FSET_FAST>(mode, lanesPerRay, cnt, floatWeights)
+                                        : views_kernel_ptr_c<FSET_PREC>(mode, lanesPerRay, cnt, floatWeights);
     if (!k) return MM_E_FORMAT;
     const unsigned grid = (unsigned)total;
     void* args[] = {(void*)&views, (void*)&nViews, (void*)&t, (void*)&total, (void*)&counts};
@@ -1464,6 +1595,17 @@ int launch_march(int fset, const MarchConsts& c, const MarchTextures& t, float* 
                  void* stream) {
     hipStream_t s = (hipStream_t)stream;
     const bool count = counts != nullptr;
+    if (c.subtexelBits == 0) {
+        switch (fset) {
+            case FSET_REF: return count ? launch_float_weights<FSET_REF | kFloatWeights, true>(c, t, out, counts, s)
+                                        : launch_float_weights<FSET_REF | kFloatWeights, false>(c, t, out, counts, s);
+            case FSET_FAST: return count ? launch_float_weights<FSET_FAST | kFloatWeights, true>(c, t, out, counts, s)
+                                         : launch_float_weights<FSET_FAST | kFloatWeights, false>(c, t, out, counts, s);
+            case FSET_PREC: return count ? launch_float_weights<FSET_PREC | kFloatWeights, true>(c, t, out, counts, s)
+                                         : launch_float_weights<FSET_PREC | kFloatWeights, false>(c, t, out, counts, s);
+        }
+        return MM_E_FORMAT;
+    }
     switch (fset) {
         case FSET_REF: return count ? launch_t<FSET_REF, true>(c, t, out, counts, s) : launch_t<FSET_REF, false>(c, t, out, counts, s);
         case FSET_FAST: return count ? launch_t<FSET_FAST, true>(c, t, out, counts, s) : launch_t<FSET_FAST, false>(c, t, out, counts, s);

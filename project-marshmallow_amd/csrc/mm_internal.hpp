@@ -123,7 +123,7 @@ struct ViewDesc {
     int tileBase;                 // global index of this view's first tile
 };
 
-int launch_views(int fset, int mode, int lanesPerRay, const ViewDesc* views, int nViews, const MarchTextures& t, int total,
+int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, const ViewDesc* views, int nViews, const MarchTextures& t, int total,
                  unsigned long long* counts, void* stream);
 
 size_t surrogate_model_bytes();
