@@ -266,7 +266,7 @@ __device__ __forceinline__ float worley_at(float px, float py, float pz, int cel
                 float dy = by - uy;
                 float dz = bz - uz;
                 float d2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-                best = d2 < best ? d2 : best;
+                best = __builtin_fminf(d2, best);  // = d2 < best ? d2 : best (no NaN: d2 >= 0)
             }
         }
     }
@@ -401,19 +401,24 @@ struct GenArgs {
 // (cells 4..128). "Level" l < 4 is Perlin octave l, l >= 4 is Worley layer (l - 4) / 3, octave
 // (l - 4) % 3; both sources below hand out the lattice data of a level.
 constexpr int kLowLevels = 16;
-__host__ __device__ __forceinline__ int low_level_cells(int l) {
+__host__ __device__ constexpr int low_level_cells(int l) {
     return l < 4 ? 4 << l : (4 << ((l - 4) / 3)) << ((l - 4) % 3);
 }
 __device__ __forceinline__ uint32_t low_level_seed(uint32_t seed, int l) {
     return l < 4 ? layer_seed(seed, 1) + (uint32_t)l : layer_seed(seed, 2 + (uint32_t)(l - 4) / 3) + (uint32_t)((l - 4) % 3);
 }
 
+// Bricks of the low-res kernel (texels per axis) and the lattice cells a block can stage in LDS.
+constexpr int kBrickX = 8, kBrickY = 8, kBrickZ = 4;
+constexpr int kStageCap = 1536;  // staged cells per block: 24 KB of LDS
+
+// Sources of the low-res levels: get<l>() hands out the lattice data of level l.
 struct LowResHash {
     uint32_t seed;
-    __device__ __forceinline__ HashGrads perlin(int o) const { return HashGrads{low_level_cells(o), low_level_seed(seed, o)}; }
-    __device__ __forceinline__ HashCells worley(int L, int k) const {
-        const int l = 4 + 3 * L + k;
-        return HashCells{low_level_cells(l), low_level_seed(seed, l)};
+    template <int L>
+    __device__ __forceinline__ auto get() const {
+        if constexpr (L < 4) return HashGrads{low_level_cells(L), low_level_seed(seed, L)};
+        else return HashCells{low_level_cells(L), low_level_seed(seed, L)};
     }
 };
 
@@ -429,16 +434,86 @@ struct LowResLds {
         const StageMeta& m = meta[l];
         return LdsCells{cells + m.off, m.bx, m.by, m.bz, m.nx, m.ny};
     }
-    __device__ __forceinline__ LdsCells perlin(int o) const { return level(o); }
-    __device__ __forceinline__ LdsCells worley(int L, int k) const { return level(4 + 3 * L + k); }
+    template <int L>
+    __device__ __forceinline__ LdsCells get() const { return level(L); }
 };
 
-template <class Src>
-__device__ __forceinline__ float worley_fbm_at(float px, float py, float pz, int L, const Src& src) {
-    const int cells = 4 << L;
-    float a = worley_at(px, py, pz, cells, src.worley(L, 0));
-    float b = worley_at(px, py, pz, cells * 2, src.worley(L, 1));
-    float c = worley_at(px, py, pz, cells * 4, src.worley(L, 2));
+// The staged boxes of a 128^3 volume. Bricks start at multiples of 8 (x, y) and 4 (z), and texel t
+// sits at q = (2t + 1) c / 256 in cell units (exact), so a brick of B texels spans
+// floor(q1) - floor(q0) = max(B c / 128 - 1, 0) cells at every position: the boxes that axis_span
+// computes have the same size for every brick, kDim128N cells per axis (pad 1 for Worley levels).
+constexpr int kDim128Span(int l, int b) { return b * low_level_cells(l) / 128 >= 1 ? b * low_level_cells(l) / 128 - 1 : 0; }
+constexpr int kDim128N(int l, int b) { return kDim128Span(l, b) + 2 + (l < 4 ? 0 : 1); }
+constexpr int kDim128Cells(int l) { return kDim128N(l, kBrickX) * kDim128N(l, kBrickY) * kDim128N(l, kBrickZ); }
+constexpr int kDim128Off(int l) { return l == 0 ? 0 : kDim128Off(l - 1) + kDim128Cells(l - 1); }
+static_assert(kDim128N(15, 8) == 10 && kDim128N(15, 4) == 6 && kDim128N(0, 8) == 2, "128^3 staged box sizes");
+static_assert(kDim128Off(kLowLevels) <= kStageCap, "a 128^3 brick's lattice cells fit the stage");
+
+// LDS cells with a compile-time box: the 27 neighbour reads of a Worley evaluation become one base
+// address plus immediate offsets.
+template <int NX, int NY>
+struct LdsCellsC {
+    const float4* cell;
+    int bx, by, bz;
+    __device__ __forceinline__ void point(int x, int y, int z, float& ux, float& uy, float& uz) const {
+        const float4 v = cell[((z - bz) * NY + (y - by)) * NX + (x - bx)];
+        ux = v.x;
+        uy = v.y;
+        uz = v.z;
+    }
+    __device__ __forceinline__ const float4* home(int x, int y, int z) const {
+        return cell + (((z - bz) * NY + (y - by)) * NX + (x - bx));
+    }
+};
+
+// worley_at for a compile-time box: the same arithmetic, the neighbour cells read at constant
+// offsets from the home cell.
+template <int NX, int NY>
+__device__ __forceinline__ float worley_at(float px, float py, float pz, int cells, const LdsCellsC<NX, NY>& src) {
+    float qx = px * (float)cells, qy = py * (float)cells, qz = pz * (float)cells;
+    int ix = (int)floorf(qx), iy = (int)floorf(qy), iz = (int)floorf(qz);
+    const float4* h = src.home(ix, iy, iz);
+    float best = 1e30f;
+#pragma unroll
+    for (int oz = -1; oz <= 1; oz++) {
+        const float bz = qz - (float)(iz + oz);
+#pragma unroll
+        for (int oy = -1; oy <= 1; oy++) {
+            const float by = qy - (float)(iy + oy);
+#pragma unroll
+            for (int ox = -1; ox <= 1; ox++) {
+                const float4 u = h[(oz * NY + oy) * NX + ox];
+                float dx = (qx - (float)(ix + ox)) - u.x;
+                float dy = by - u.y;
+                float dz = bz - u.z;
+                float d2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+                best = __builtin_fminf(d2, best);
+            }
+        }
+    }
+    float d = sqrt_rn(best);
+    return 1.0f - (d < 1.0f ? d : 1.0f);
+}
+
+struct LowResLds128 {
+    const float4* cells;
+    const StageMeta* meta;
+    template <int L>
+    __device__ __forceinline__ LdsCellsC<kDim128N(L, kBrickX), kDim128N(L, kBrickY)> get() const {
+        // wave-uniform box origin (the brick's), in scalar registers
+        const int bx = __builtin_amdgcn_readfirstlane(meta[L].bx), by = __builtin_amdgcn_readfirstlane(meta[L].by),
+                  bz = __builtin_amdgcn_readfirstlane(meta[L].bz);
+        constexpr int off = kDim128Off(L);  // evaluated here, not by a device-side recursive call
+        return {cells + off, bx, by, bz};
+    }
+};
+
+template <int LY, class Src>
+__device__ __forceinline__ float worley_fbm_at(float px, float py, float pz, const Src& src) {
+    constexpr int cells = 4 << LY;
+    float a = worley_at(px, py, pz, cells, src.template get<4 + 3 * LY>());
+    float b = worley_at(px, py, pz, cells * 2, src.template get<5 + 3 * LY>());
+    float c = worley_at(px, py, pz, cells * 4, src.template get<6 + 3 * LY>());
     return (0.625f * a + 0.25f * b) + 0.125f * c;
 }
 
@@ -447,19 +522,21 @@ __device__ __forceinline__ float worley_fbm_at(float px, float py, float pz, int
 template <class Src>
 __device__ __forceinline__ void low_res_values(float px, float py, float pz, const Src& src, float v[4]) {
     float s = 0.f, w = 1.f;
-#pragma unroll
-    for (int o = 0; o < 4; o++) {
-        s = s + w * perlin_at(px, py, pz, 4 << o, src.perlin(o));
-        w = w * 0.5f;
-    }
+    s = s + w * perlin_at(px, py, pz, 4, src.template get<0>());
+    w = w * 0.5f;
+    s = s + w * perlin_at(px, py, pz, 8, src.template get<1>());
+    w = w * 0.5f;
+    s = s + w * perlin_at(px, py, pz, 16, src.template get<2>());
+    w = w * 0.5f;
+    s = s + w * perlin_at(px, py, pz, 32, src.template get<3>());
     const float perlin = s * (1.0f / 1.875f);
     float p01 = clamp01(0.5f + 1.5f * perlin);
-    float wf = worley_fbm_at(px, py, pz, 0, src);
+    float wf = worley_fbm_at<0>(px, py, pz, src);
     float pw = (p01 - (wf - 1.0f)) / (1.0f - (wf - 1.0f));
     v[0] = clamp01(0.8f * pw - 0.04f);
-    v[1] = clamp01(0.26f + 0.9f * worley_fbm_at(px, py, pz, 1, src));
-    v[2] = clamp01(0.26f + 0.9f * worley_fbm_at(px, py, pz, 2, src));
-    v[3] = clamp01(0.26f + 0.9f * worley_fbm_at(px, py, pz, 3, src));
+    v[1] = clamp01(0.26f + 0.9f * worley_fbm_at<1>(px, py, pz, src));
+    v[2] = clamp01(0.26f + 0.9f * worley_fbm_at<2>(px, py, pz, src));
+    v[3] = clamp01(0.26f + 0.9f * worley_fbm_at<3>(px, py, pz, src));
 }
 
 __device__ __forceinline__ void store_low_res(const GenArgs& a, long i, const float v[4]) {
@@ -478,8 +555,6 @@ __device__ __forceinline__ void store_low_res(const GenArgs& a, long i, const fl
 // touches more cells than LDS holds (small volumes, where cells outnumber texels) it hashes in
 // place. Texels whose coordinate is below the apron width are also written to the apron copy
 // (x + dim, ...).
-constexpr int kBrickX = 8, kBrickY = 8, kBrickZ = 4;
-constexpr int kStageCap = 1536;  // staged cells per block: 24 KB of LDS
 
 // Cells [floor(q(t0)) - pad, floor(q(t1)) + 1] of one axis, q(t) = ((2t + 1) / (2 dim)) * cells,
 // evaluated as the texel evaluates it (q is monotone in t, so every texel's cells lie inside).
@@ -490,6 +565,8 @@ __device__ __forceinline__ void axis_span(int t0, int t1, float inv, int cells, 
     n = ((int)floorf(q1) + 1) - base + 1;
 }
 
+// DIM128: the volume is 128^3 (asserted by the host), whose staged boxes have compile-time sizes.
+template <bool DIM128>
 __global__ __launch_bounds__(256) void low_res_brick_kernel(GenArgs a) {
     __shared__ float4 cells[kStageCap];
     __shared__ StageMeta meta[kLowLevels];
@@ -521,7 +598,7 @@ __global__ __launch_bounds__(256) void low_res_brick_kernel(GenArgs a) {
     }
     __syncthreads();
     const int nst = total;
-    const bool staged = nst <= kStageCap;  // uniform across the block
+    const bool staged = DIM128 || nst <= kStageCap;  // uniform across the block
     if (staged) {
         for (int e = tid; e < nst; e += 256) {
             int l = 0;
@@ -541,7 +618,9 @@ __global__ __launch_bounds__(256) void low_res_brick_kernel(GenArgs a) {
     if (x > x1 || y > y1 || z > z1) return;
     const float px = (float)(2 * x + 1) * inv, py = (float)(2 * y + 1) * inv, pz = (float)(2 * z + 1) * inv;
     float v[4];
-    if (staged)
+    if constexpr (DIM128)
+        low_res_values(px, py, pz, LowResLds128{cells, meta}, v);
+    else if (staged)
         low_res_values(px, py, pz, LowResLds{cells, meta}, v);
     else
         low_res_values(px, py, pz, LowResHash{a.seed}, v);
@@ -807,7 +886,10 @@ int launch_generate(const mm_noise_params* p, const mm_noise_set* out, void* wor
             for (int k = 0; k < 4; k++) a.inv_scale[k] = 1.0f / t.scale[k];
         const long bricks = (long)((t.width + kBrickX - 1) / kBrickX) * ((t.width + kBrickY - 1) / kBrickY) *
                             ((t.width + kBrickZ - 1) / kBrickZ);
-        hipLaunchKernelGGL(low_res_brick_kernel, dim3((unsigned)bricks), dim3(256), 0, main, a);
+        if (t.width == 128)
+            hipLaunchKernelGGL(low_res_brick_kernel<true>, dim3((unsigned)bricks), dim3(256), 0, main, a);
+        else
+            hipLaunchKernelGGL(low_res_brick_kernel<false>, dim3((unsigned)bricks), dim3(256), 0, main, a);
     }
     if (side) {
         const int e = side_join(main, side);

@@ -353,3 +353,22 @@ def test_generated_statistics_match_the_reference_assets(orc):
     assert abs(w[:, 0].mean() - wm[0]) < 0.08 and abs(w[:, 2].mean() - wm[2]) < 0.08
     corr = np.corrcoef(w[:, 0], w[:, 1])[0, 1]
     assert abs(corr - st["weather"]["corr_rg"]) < 0.2
+
+
+def test_low_res_staged_boxes_of_a_128_volume_are_fixed():
+    """The 128^3 low-res kernel reads its staged lattice cells at compile-time offsets: every brick's box
+    (axis_span in noise.hip: [floor(q(t0)) - pad, floor(q(t1)) + 1], q(t) = ((2t + 1) / 256) c in float32)
+    must have the size max(B c / 128 - 1, 0) + 2 + pad for B texels per brick axis (8, 8, 4)."""
+    dim = 128
+    inv = np.float32(1.0 / (2 * dim))
+    for level in range(16):
+        cells = 4 << level if level < 4 else (4 << ((level - 4) // 3)) << ((level - 4) % 3)
+        pad = 0 if level < 4 else 1
+        for b in (8, 4):
+            t0 = np.arange(0, dim, b)
+            t1 = t0 + b - 1
+            q0 = (((2 * t0 + 1).astype(np.float32) * inv) * np.float32(cells)).astype(np.float32)
+            q1 = (((2 * t1 + 1).astype(np.float32) * inv) * np.float32(cells)).astype(np.float32)
+            n = (np.floor(q1).astype(int) + 1) - (np.floor(q0).astype(int) - pad) + 1
+            want = max(b * cells // 128 - 1, 0) + 2 + pad
+            assert (n == want).all(), (level, b, set(n.tolist()), want)
