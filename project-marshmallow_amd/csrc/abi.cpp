@@ -86,6 +86,8 @@ mm::TexView view_of(const mm_texture& t) {
     v.stride = t.width + a;
     v.sliceStride = (t.width + a) * (t.height + a);
     for (int k = 0; k < 4; k++) v.scale[k] = t.scale[k];
+    v.lw = is_pow2(t.width) ? __builtin_ctz((unsigned)t.width) : 0;
+    v.lh = is_pow2(t.height) ? __builtin_ctz((unsigned)t.height) : 0;
     return v;
 }
 

@@ -139,15 +139,18 @@ __device__ __forceinline__ float remapClamped01(float v, float a) {
 // k = floor(x * 256 + 0.5) (x * 256 exact) gives floor(k / 256) = k >> 8 and the fraction
 // (k & 255) / 256 exactly as the float steps do, for texel coordinates below 2^23 in magnitude
 // (8.4 M texels; the presets stay below 1e5). With the rounding, w = (256 - fk) | fk << 16 holds the
-// integer weights of the two texels (fk = k & 255) for the integer filter below.
-__device__ __forceinline__ void axis_p2(float u, int n, int q8, int& i0, float& f, uint32_t& w) {
+// integer weights of the two texels (fk = k & 255) for the integer filter below. k comes from one
+// v_cvt_flr_i32_f32 (floor and conversion), i0 from one v_bfe_u32 (bits 8 .. 8 + log2 n of k, the
+// same as (k >> 8) & (n - 1) in two's complement); lg = log2 n.
+__device__ __forceinline__ void axis_p2(float u, int n, int lg, int q8, int& i0, float& f, uint32_t& w) {
     const float x = __builtin_fmaf(u, (float)n, -0.5f);
     if (q8) {
-        const int k = (int)floorf(__builtin_fmaf(x, 256.0f, 0.5f));
-        i0 = (k >> 8) & (n - 1);
+        int k;
+        asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(k) : "v"(__builtin_fmaf(x, 256.0f, 0.5f)));
+        i0 = (int)__builtin_amdgcn_ubfe((uint32_t)k, 8u, (uint32_t)lg);
         const uint32_t fk = (uint32_t)k & 255u;
         f = (float)fk * (1.0f / 256.0f);
-        w = (256u - fk) | (fk << 16);
+        w = __umul24(fk, 65535u) + 256u;  // (256 - fk) | fk << 16
     } else {
         const float fl = floorf(x);
         f = x - fl;
@@ -216,8 +219,8 @@ __device__ __forceinline__ G2<BYTES> gather2(const TexView& tv, float u, float v
     G2<BYTES> g;
     g.q8 = q8;
     int x0, y0;
-    axis_p2(u, tv.w, q8, x0, g.fx, g.wx);
-    axis_p2(v, tv.h, q8, y0, g.fy, g.wy);
+    axis_p2(u, tv.w, tv.lw, q8, x0, g.fx, g.wx);
+    axis_p2(v, tv.h, tv.lh, q8, y0, g.fy, g.wy);
     int x1, y1;
     if (APRON) { x1 = x0 + 1; y1 = y0 + 1; }
     else { x1 = (x0 + 1) & (tv.w - 1); y1 = (y0 + 1) & (tv.h - 1); }
@@ -243,9 +246,10 @@ __device__ __forceinline__ G3<BYTES> gather3(const TexView& tv, float u, float v
     g.q8 = q8;
     int x0, y0, z0;
     const int n = tv.w;
-    axis_p2(u, n, q8, x0, g.fx, g.wx);
-    axis_p2(v, n, q8, y0, g.fy, g.wy);
-    axis_p2(w, n, q8, z0, g.fz, g.wz);
+    axis_p2(u, n, tv.lw, q8, x0, g.fx, g.wx);
+    axis_p2(v, n, tv.lw, q8, y0, g.fy, g.wy);
+    axis_p2(w, n, tv.lw, q8, z0, g.fz, g.wz);
+
     int x1, y1, z1;
     if (APRON) { x1 = x0 + 1; y1 = y0 + 1; z1 = z0 + 1; }
     else { x1 = (x0 + 1) & (n - 1); y1 = (y0 + 1) & (n - 1); z1 = (z0 + 1) & (n - 1); }

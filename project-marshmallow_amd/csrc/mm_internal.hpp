@@ -63,6 +63,7 @@ struct TexView {
     int stride;                  // texels per row including the apron (w or w + 1)
     int sliceStride;             // texels per slice including the apron
     float scale[4];
+    int lw, lh;                  // log2 of w and h for power-of-two dims (else 0)
 };
 
 enum FormatSet { FSET_REF = 0, FSET_FAST = 1, FSET_PREC = 2 };
