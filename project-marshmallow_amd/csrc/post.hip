@@ -1,7 +1,7 @@
 // Post chain for gfx950: the HIP counterpart of the offscreen passes that consume the cloud image,
 // god-ray.frag:41-76 -> radialBlur.frag:36-63 -> tonemap.frag:11-34 (VulkanApplication.cpp:938-966).
 //
-// Three launches of 16x16-pixel blocks (four wave64 rows of 16x4):
+// Three launches of 16x16-pixel blocks (four wave64 rows of 16x4), one contiguous band of tiles per XCD:
 //   alpha_plane_kernel  the cloud image's alpha as a plane of floats (16 B read, 4 B written);
 //   god_ray_kernel      one work-item per pixel: 9 bilinear taps of that plane (centre + 8 toward
 //                       the sun), writes the alpha plane of framebuffer 1 (4 B/pixel);
@@ -9,9 +9,10 @@
 //                       taps of that alpha plane toward the sun, the radial-blur colour, then the
 //                       Uncharted-2 tonemap + vignette, written as RGBA8 (and optionally the HDR
 //                       radial-blur value).
-// The taps of neighbouring pixels overlap, so after the first touch they come from L2: the passes
-// are bound by HBM at 16 B read + 4 B written (alpha plane), 4 B read + 4 B written (god rays) and
-// 16 + 4 B read + 4 (or 20) B written (radial blur + tonemap).
+// The taps of neighbouring pixels overlap, so after the first touch they come from L2. The algorithmic
+// traffic is 16 B read + 4 B written (alpha plane), 4 B read + 4 B written (god rays) and 16 + 4 B
+// read + 4 (or 20) B written (radial blur + tonemap) per pixel; the two tap kernels issue more VALU
+// work per pixel (9 and 11 bilinear taps, exact division) than that traffic takes at HBM rate.
 // Arithmetic follows the shaders' evaluation order with IEEE division/sqrt and no contraction; the
 // sampler and the other choices GLSL leaves open are the scalar restatement's, so the result is the
 // same bit for bit.
@@ -122,6 +123,21 @@ __device__ __forceinline__ float exp2_series(float t) {
     return __builtin_amdgcn_ldexpf(p * 1.41421356f, (int)n);
 }
 
+// Block -> 16x16 tile. Workgroups are dealt round-robin to the 8 XCDs (block b runs on XCD b mod 8),
+// each with its own L2; the taps of a pixel reach far toward the sun, so tiles that share cache
+// lines should share an XCD. Block b takes tile (b mod 8) * per + b / 8 of the row-major tile list
+// (per = ceil(tiles / 8)): each XCD marches one contiguous band of tile rows. Returns false for the
+// padding blocks past the last tile.
+__device__ __forceinline__ bool xcd_tile(const mm::PostConsts& c, int& x, int& y) {
+    const int gx = (c.width + 15) >> 4, tiles = gx * ((c.height + 15) >> 4);
+    const int per = (tiles + 7) >> 3;
+    const int t = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    if (t >= tiles) return false;
+    x = (t % gx) * 16 + (threadIdx.x & 15);
+    y = (t / gx) * 16 + (threadIdx.x >> 4);
+    return true;
+}
+
 __device__ __forceinline__ uint32_t unorm8(float v) {
     if (!(v == v)) return 0u;
     return (uint32_t)floorf(clampf(v, 0.0f, 1.0f) * 255.0f + 0.5f);
@@ -131,16 +147,16 @@ __device__ __forceinline__ uint32_t unorm8(float v) {
 // 128-byte line instead of 8 RGBA32F ones), which is what keeps their far-reaching taps in cache.
 __global__ __launch_bounds__(256) void alpha_plane_kernel(const mm::PostConsts c, const float4* __restrict__ img,
                                                           float* __restrict__ plane) {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= c.width || y >= c.height) return;
+    int x, y;
+    if (!xcd_tile(c, x, y) || x >= c.width || y >= c.height) return;
     const uint32_t i = (uint32_t)y * (uint32_t)c.pitch + (uint32_t)x;
     plane[i] = img[i].w;
 }
 
 __global__ __launch_bounds__(256) void god_ray_kernel(const mm::PostConsts c, const float* __restrict__ a0,
                                                       float* __restrict__ alpha) {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= c.width || y >= c.height) return;
+    int x, y;
+    if (!xcd_tile(c, x, y) || x >= c.width || y >= c.height) return;
     float a = 1.0f;  // sun below the horizon: the pass writes alpha 1 (god-ray.frag:55-57)
     if (!c.below) {
         const float u = __fdiv_rn((float)x + 0.5f, (float)c.width), v = __fdiv_rn((float)y + 0.5f, (float)c.height);
@@ -166,8 +182,8 @@ __constant__ float kBlurTaps[10] = {-0.08f, -0.05f, -0.03f, -0.02f, -0.01f, 0.01
 __global__ __launch_bounds__(256) void blur_tonemap_kernel(const mm::PostConsts c, const float4* __restrict__ img,
                                                            const float* __restrict__ alpha, float4* __restrict__ hdr,
                                                            uint32_t* __restrict__ rgba8) {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= c.width || y >= c.height) return;
+    int x, y;
+    if (!xcd_tile(c, x, y) || x >= c.width || y >= c.height) return;
     const float u = __fdiv_rn((float)x + 0.5f, (float)c.width), v = __fdiv_rn((float)y + 0.5f, (float)c.height);
     float rgb[3];
     tap_rgb(img, c, u, v, rgb);
@@ -211,7 +227,8 @@ namespace mm {
 
 int launch_post(const PostConsts& c, const float* image, float* scratch, float* hdr, uint8_t* rgba8, void* stream) {
     hipStream_t s = (hipStream_t)stream;
-    dim3 grid((unsigned)((c.width + 15) / 16), (unsigned)((c.height + 15) / 16)), block(256);
+    const long tiles = (long)((c.width + 15) / 16) * ((c.height + 15) / 16);
+    dim3 grid((unsigned)(8 * ((tiles + 7) / 8))), block(256);  // 1-D: xcd_tile maps blocks to tiles
     // scratch: plane 0 = the image's alpha, plane 1 = framebuffer 1's alpha (the god-ray output)
     float* a0 = scratch;
     float* a1 = scratch + (size_t)c.pitch * c.height;
