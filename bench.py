@@ -68,6 +68,18 @@ def valu_from_profiles(fset, width, height):
     return None
 
 
+def valu_live(p, kernel_ms):
+    """The VALU-issue axis of the march at this run's kernel time: the profiled VALU instructions per
+    launch over the time, against the issue rate of 1024 SIMDs at the profiled clock (one wave64 VALU
+    instruction per SIMD every 4 cycles; dual issue can exceed it)."""
+    if not p or "insts_per_launch" not in p or "clock_mhz" not in p:
+        return p
+    achieved = p["insts_per_launch"] / (kernel_ms * 1e-3) / 1e9
+    peak = SIMDS * p["clock_mhz"] * 1e6 / 4 / 1e9
+    return dict(p, achieved=round(achieved, 1), peak=round(peak, 1), unit="G wave-instructions/s",
+                frac=round(achieved / peak, 4))
+
+
 def cpu_baseline(mm, ns, cam, sun, sky, prm, W, H, budget_s):
     """FP32 oracle (all host threads) on whole 1/16 pixel phases of the same frame until the budget
     is spent; returns density samples/s."""
@@ -399,7 +411,7 @@ def main():
                 "gathered": {"bytes_per_launch": gathered, "achieved": round(gathered / (march_ms * 1e-3) / 1e9, 1),
                              "frac": round(gathered / (march_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                              "weather_reads": n_weather, "low_res_reads": n_low},
-                "valu": valu_from_profiles(args.fset, W, H)}
+                "valu": valu_live(valu_from_profiles(args.fset, W, H), march_ms)}
     rec = {
         "metric": METRIC,
         "value": round(n_density / (ms_step * 1e-3), 1),
