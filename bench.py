@@ -259,7 +259,9 @@ def sweep_leg(mm, args, prm, W, H):
     """The 1920x1080 workload over its scene sweep (coverage x sun elevation x wind time x weather
     seed, 225 scenes): per scene one step = every noise texture regenerated + the full-frame march,
     timed like the headline step (HIP events, args.steps steps after args.warmup), the sample counts
-    from one instrumented launch. Prints one JSON line per scene and returns the summary."""
+    from one instrumented launch. Both roofline axes per scene: the HBM one from the scene's own
+    sample counts, the VALU one estimated from the profiled headline scene's instructions per sample
+    (density + detail) times the scene's samples. Prints one JSON line per scene and returns the summary."""
     import itertools
 
     import torch
@@ -268,6 +270,7 @@ def sweep_leg(mm, args, prm, W, H):
     out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
     stream = mm.current_stream()
     b = BYTES_PER_SAMPLE[args.fset]
+    vp = valu_from_profiles(args.fset, W, H)
     rows = []
     for cov, sun_deg, t, seed in itertools.product(*SWEEP.values()):
         cam, sun, sky = mm.build_ubos(mm.scenario_scene(altitude=200.0, pitch_deg=20.0, fov_deg=60.0, aspect=W / H,
@@ -297,16 +300,21 @@ def sweep_leg(mm, args, prm, W, H):
                "density_samples_per_s": round(nd / (frame * 1e-3), 1),
                "roofline_hbm": {"achieved_gbs": round(alg / (march * 1e-3) / 1e9, 1),
                                 "frac": round(alg / (march * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+        if vp and "samples_per_launch" in vp:
+            est = valu_live(dict(vp, insts_per_launch=vp["insts_per_launch"] * (nd + nh) / vp["samples_per_launch"]), march)
+            row["roofline_valu_est"] = {"insts_per_launch": round(est["insts_per_launch"]), "frac": est["frac"]}
         rows.append(row)
         print(json.dumps({"sweep_scene": row}), flush=True)
     ms = sorted(r["ms_per_frame"] for r in rows)
     sps = sorted(r["density_samples_per_s"] for r in rows)
     fr = sorted(r["roofline_hbm"]["frac"] for r in rows)
+    fv = sorted(r["roofline_valu_est"]["frac"] for r in rows if "roofline_valu_est" in r)
     pick = lambda v, q: v[min(len(v) - 1, int(q * (len(v) - 1) + 0.5))]
     return {"scenes": len(rows), "axes": {k: list(v) for k, v in SWEEP.items()},
             "ms_per_frame": {"min": ms[0], "median": pick(ms, 0.5), "max": ms[-1]},
             "density_samples_per_s": {"min": sps[0], "median": pick(sps, 0.5), "max": sps[-1]},
-            "roofline_hbm_frac": {"min": fr[0], "median": pick(fr, 0.5), "max": fr[-1]}}
+            "roofline_hbm_frac": {"min": fr[0], "median": pick(fr, 0.5), "max": fr[-1]},
+            "roofline_valu_frac_est": {"min": fv[0], "median": pick(fv, 0.5), "max": fv[-1]} if fv else None}
 
 
 def main():
