@@ -129,7 +129,8 @@ __device__ __forceinline__ float remapk(float v, float a, float inv_ba, float c,
 // sits behind a branch that whole waves of empty samples skip.
 __device__ __forceinline__ float remapClamped01(float v, float a) {
     if (!(v > a)) return 0.0f;
-    return clampf(0.0f + ((__fdiv_rn(v - a, 1.0f - a)) * 1.0f), 0.0f, 1.0f);
+    // v > a: the quotient is positive, so the shader's 0 + (...) * 1 leaves it unchanged
+    return clampf(__fdiv_rn(v - a, 1.0f - a), 0.0f, 1.0f);
 }
 
 // ------------------------------------------------------------------ sampler
@@ -184,8 +185,10 @@ template <int BYTES>
 __device__ __forceinline__ void ld_pair(const uint8_t* p, typename TexelT<BYTES>::type& a,
                                         typename TexelT<BYTES>::type& b) {
     if constexpr (BYTES == 2) {
+        // a keeps the whole word (texel x0 in bytes 0-1, x0 + 1 in bytes 2-3): the byte decoders only
+        // read bytes 0-1 of it, and the integer filter takes both texels' channels from it directly
         const uint32_t v = *reinterpret_cast<const u32_a2*>(p);
-        a = v & 0xffffu;
+        a = v;
         b = v >> 16;
     } else if constexpr (BYTES == 4) {
         const u64_a4 v = *reinterpret_cast<const u64_a4*>(p);
@@ -312,10 +315,22 @@ __device__ __forceinline__ uint32_t pair8(uint32_t a, uint32_t b) {
 // (low half of a, low half of b) as a 16-bit pair
 __device__ __forceinline__ uint32_t pair16(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x05040100u); }
 
+// (byte K, byte 2 + K) of one word as a 16-bit pair: channel K of the two 2-byte texels of an apron row pair
+template <int K>
+__device__ __forceinline__ uint32_t pair8w(uint32_t w) {
+    return __builtin_amdgcn_perm(0u, w, (0x0cu << 24) | ((2u + (uint32_t)K) << 16) | (0x0cu << 8) | (uint32_t)K);
+}
+
 template <int K, int BYTES>
 __device__ __forceinline__ uint32_t isum2(const G2<BYTES>& g) {
-    const uint32_t l0 = dot2u(pair8<K>(g.t[0], g.t[1]), g.wx, 0u);
-    const uint32_t l1 = dot2u(pair8<K>(g.t[2], g.t[3]), g.wx, 0u);
+    uint32_t l0, l1;
+    if constexpr (BYTES == 2) {  // apron gather: t[0] and t[2] are whole row-pair words (ld_pair)
+        l0 = dot2u(pair8w<K>(g.t[0]), g.wx, 0u);
+        l1 = dot2u(pair8w<K>(g.t[2]), g.wx, 0u);
+    } else {
+        l0 = dot2u(pair8<K>(g.t[0], g.t[1]), g.wx, 0u);
+        l1 = dot2u(pair8<K>(g.t[2], g.t[3]), g.wx, 0u);
+    }
     return dot2u(pair16(l0, l1), g.wy, 0u);
 }
 template <int K, int BYTES>
@@ -392,18 +407,21 @@ __device__ __forceinline__ Weather read_weather(const TexView& tv, float u, floa
     return w;
 }
 
+// Twice the filtered curl texel (the 2 cv of comp:218's 2 cv - 1). Doubling commutes with rounding for
+// these values (no overflow or underflow), so the 2 folds into the scale: 2 RN(x s) = RN(x (2 s)).
 template <int FSET>
-__device__ __forceinline__ V3 read_curl(const TexView& tv, float u, float v, int q8) {
+__device__ __forceinline__ V3 read_curl2(const TexView& tv, float u, float v, int q8) {
     if constexpr (kSet<FSET> == mm::FSET_PREC) {
         G2<8> g = gather2<8, true>(tv, u, v, q8);
-        return mk(filt2(g, Half4<0>()), filt2(g, Half4<1>()), filt2(g, Half4<2>()));
+        return mk(2.0f * filt2(g, Half4<0>()), 2.0f * filt2(g, Half4<1>()), 2.0f * filt2(g, Half4<2>()));
     } else {
         G2<4> g = gather2<4, kSet<FSET> != mm::FSET_REF>(tv, u, v, q8);
+        constexpr float s16 = 2.0f * kInv255 * 0x1p-16f, s = 2.0f * kInv255;
         if (q8) {
             const uint32_t x = isum2<0>(g), y = isum2<1>(g), z = isum2<2>(g);
-            return mk((float)x * (kInv255 * 0x1p-16f), (float)y * (kInv255 * 0x1p-16f), (float)z * (kInv255 * 0x1p-16f));
+            return mk((float)x * s16, (float)y * s16, (float)z * s16);
         }
-        return mk(filt2(g, Byte<0>()) * kInv255, filt2(g, Byte<1>()) * kInv255, filt2(g, Byte<2>()) * kInv255);
+        return mk(filt2(g, Byte<0>()) * s, filt2(g, Byte<1>()) * s, filt2(g, Byte<2>()) * s);
     }
 }
 
@@ -416,18 +434,20 @@ __device__ __forceinline__ float read_hi_erosion(const TexView& tv, V3 p, int q8
         return ((0.625f * r) + (0.25f * gg)) + (0.125f * b);
     } else {
         G3<4> g = gather3<4, kSet<FSET> != mm::FSET_REF>(tv, p.x, p.y, p.z, q8);
-        float r, gg, b;
+        // the power-of-two weights fold into the channel scale: RN(x s) / 4 = RN(x (s / 4)) for these
+        // normal-range values
+        float r, gg4, b8;
         if (q8) {
             const uint32_t nr = isum3<0>(g), ng = isum3<1>(g), nb = isum3<2>(g);
             r = (float)nr * (kInv255 * 0x1p-24f);
-            gg = (float)ng * (kInv255 * 0x1p-24f);
-            b = (float)nb * (kInv255 * 0x1p-24f);
+            gg4 = (float)ng * (0.25f * kInv255 * 0x1p-24f);
+            b8 = (float)nb * (0.125f * kInv255 * 0x1p-24f);
         } else {
             r = filt3(g, Byte<0>()) * kInv255;
-            gg = filt3(g, Byte<1>()) * kInv255;
-            b = filt3(g, Byte<2>()) * kInv255;
+            gg4 = filt3(g, Byte<1>()) * (0.25f * kInv255);
+            b8 = filt3(g, Byte<2>()) * (0.125f * kInv255);
         }
-        return ((0.625f * r) + (0.25f * gg)) + (0.125f * b);
+        return ((0.625f * r) + gg4) + b8;
     }
 }
 
@@ -455,18 +475,18 @@ __device__ __forceinline__ float low_erosion(const LowG<FSET>& l, const TexView&
               a = filt3(l.g, Fp8<3>()) * tv.scale[3];
         return ((0.625f * g) + (0.25f * b)) + (0.125f * a);
     } else {
-        float g, b, a;
+        float g, b4, a8;  // power-of-two weights folded into the scale, as in read_hi_erosion
         if (l.g.q8) {
             const uint32_t ng = isum3<1>(l.g), nb = isum3<2>(l.g), na = isum3<3>(l.g);
             g = (float)ng * (kInv255 * 0x1p-24f);
-            b = (float)nb * (kInv255 * 0x1p-24f);
-            a = (float)na * (kInv255 * 0x1p-24f);
+            b4 = (float)nb * (0.25f * kInv255 * 0x1p-24f);
+            a8 = (float)na * (0.125f * kInv255 * 0x1p-24f);
         } else {
             g = filt3(l.g, Byte<1>()) * kInv255;
-            b = filt3(l.g, Byte<2>()) * kInv255;
-            a = filt3(l.g, Byte<3>()) * kInv255;
+            b4 = filt3(l.g, Byte<2>()) * (0.25f * kInv255);
+            a8 = filt3(l.g, Byte<3>()) * (0.125f * kInv255);
         }
-        return ((0.625f * g) + (0.25f * b)) + (0.125f * a);
+        return ((0.625f * g) + b4) + a8;
     }
 }
 
@@ -483,10 +503,13 @@ __device__ __forceinline__ float rel_height(const MarchConsts& c, V3 pt, V3 proj
 // cloudLayerDensity (comp:193-204)
 __device__ __forceinline__ float layer_density(float h, float type) {
     h = clampf(h, 0.0f, 1.0f);
-    const float r02 = remapk(h, 0.0f, 1.0f / 0.2f, 0.0f, 1.0f);
+    // remap(h, 0, b, 0, 1) = 0 + h (1 / b): the 0 + only turns a -0 into +0, and these factors meet
+    // only products, max(0, .) and mixes whose zero signs never reach a nonzero profile (the caller
+    // tests the profile against 0)
+    const float r02 = h * (1.0f / 0.2f);
     float cumulus = fmaxf(0.0f, r02 * remapk(h, 0.7f, 1.0f / (0.9f - 0.7f), 1.0f, -1.0f));
     float strato = fmaxf(0.0f, r02 * remapk(h, 0.2f, 1.0f / (0.7f - 0.2f), 1.0f, -1.0f));
-    float stratus = fmaxf(0.0f, remapk(h, 0.0f, 1.0f / 0.1f, 0.0f, 1.0f) * remapk(h, 0.2f, 1.0f / (0.3f - 0.2f), 1.0f, -1.0f));
+    float stratus = fmaxf(0.0f, (h * (1.0f / 0.1f)) * remapk(h, 0.2f, 1.0f / (0.3f - 0.2f), 1.0f, -1.0f));
     float d1 = mixf(stratus, strato, clampf(type * 2.0f, 0.0f, 1.0f));
     float d2 = mixf(strato, cumulus, clampf((type - 0.5f) * 2.0f, 0.0f, 1.0f));
     return mixf(d1, d2, type);
@@ -494,7 +517,8 @@ __device__ __forceinline__ float layer_density(float h, float type) {
 
 // wind offset (comp:414, 445)
 __device__ __forceinline__ V3 wind_offset(const MarchConsts& c, float h) {
-    V3 w = mk(c.wind[0] + h * 0.1f, c.wind[1] + h * 0.05f, c.wind[2] + h * 0.0f);
+    // h is a clamped relative height in [0, 1], so h * 0 is a zero
+    V3 w = mk(c.wind[0] + h * 0.1f, c.wind[1] + h * 0.05f, c.wind[2] + 0.0f);
     return (c.windStrength * w) * (c.time + h * 200.0f);
 }
 
@@ -521,7 +545,9 @@ __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTex
     if (COUNT) nd += kTallyLow;
     LowG<FSET> lg = gather_low<FSET>(tx.low, pos * c.lowScale, q8);
     float shape = low_shape<FSET>(lg, tx.low);
-    float density = layer * clampf(0.0f + (((shape - 0.3f) * (1.0f / (1.0f - 0.3f))) * 1.0f), 0.0f, 1.0f);
+    // remapClamped(shape, 0.3, 1, 0, 1); its 0 + (.) only flips the sign of a zero, which the
+    // < 0.0001 test below treats alike
+    float density = layer * clampf((shape - 0.3f) * (1.0f / (1.0f - 0.3f)), 0.0f, 1.0f);
     if (density < 0.0001f) return 0.0f;
     // remapClamped(density, erosion, 1, 0, 1) with erosion in [0, 1] is exactly 1 for density > 1:
     // (density - erosion) and (1 - erosion) round monotonically, so their quotient is >= 1 (or +inf
@@ -541,8 +567,8 @@ __device__ __forceinline__ float cloud_hires(const MarchConsts& c, const MarchTe
                                              float orig, float h, int& nh) {
     if (COUNT) nh++;
     constexpr int q8 = kQ8<FSET>;
-    V3 cv = read_curl<FSET>(tx.curl, c.curlScale * pos.x, c.curlScale * pos.z, q8);
-    V3 curl = mk(2.0f * cv.x - 1.0f, 2.0f * cv.y - 1.0f, 2.0f * cv.z - 1.0f);
+    V3 cv2 = read_curl2<FSET>(tx.curl, c.curlScale * pos.x, c.curlScale * pos.z, q8);
+    V3 curl = mk(cv2.x - 1.0f, cv2.y - 1.0f, cv2.z - 1.0f);
     pos = pos + ((1.9f * curlStrength) * curl);
     float e = read_hi_erosion<FSET>(tx.hi, pos * c.hiScale, q8);
     e = mixf(e, 1.0f - e, clampf(h * 10.0f, 0.0f, 1.0f));
