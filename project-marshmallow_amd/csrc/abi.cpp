@@ -828,6 +828,7 @@ int mm_post_process(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const fl
 struct mm_view_plan {
     int n_views = 0, total_tiles = 0, fset = 0, mode = 0, lanes = 1;
     bool float_weights = false;       // subtexel_bits 0: the float-sampler kernels
+    bool bounded_steps = false;       // the kernels' specialised step checks (mm::bounded_steps)
     mm::ViewDesc* d_views = nullptr;  // device: ViewDesc[n_views]
     mm::MarchTextures shared;
     mm_noise_set shared_set;          // the shared textures as given (curl, low_res, hi_res)
@@ -902,6 +903,8 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
     p->fset = fset;
     p->mode = params->cone_mode;
     p->float_weights = params->subtexel_bits == 0;
+    p->bounded_steps = mm::bounded_steps(params->flags, params->max_steps, params->max_coarse_steps,
+                                         params->max_fine_steps);
     p->shared = tx;
     p->shared_set = *shared;
     p->tile_base.resize((size_t)n_views);
@@ -938,6 +941,9 @@ int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_view
     }
     if (params->cone_mode != plan->mode) return MM_E_PARAM;
     if ((params->subtexel_bits == 0) != plan->float_weights) return MM_E_PARAM;  // a different kernel
+    if (mm::bounded_steps(params->flags, params->max_steps, params->max_coarse_steps, params->max_fine_steps) !=
+        plan->bounded_steps)
+        return MM_E_PARAM;  // a different kernel
     // the shared volumes stay those of the plan; views bring new uniforms, weather maps and outputs
     std::vector<mm::ViewDesc> h;
     mm::MarchTextures tx;
@@ -970,8 +976,8 @@ int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* s
     if (!plan) return MM_E_NULL;
     if (plan->total_tiles == 0) return MM_OK;
     reset_error_state();
-    return mm::launch_views(plan->fset, plan->mode, plan->lanes, plan->float_weights, plan->d_views, plan->n_views,
-                            plan->shared, plan->total_tiles, counts, stream);
+    return mm::launch_views(plan->fset, plan->mode, plan->lanes, plan->float_weights, plan->bounded_steps,
+                            plan->d_views, plan->n_views, plan->shared, plan->total_tiles, counts, stream);
 }
 
 void mm_view_plan_destroy(mm_view_plan* plan) {

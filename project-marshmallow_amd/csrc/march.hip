@@ -27,8 +27,14 @@ namespace {
 // with kFloatWeights for the float sampler (subtexel_bits 0). Without the flag the sub-texel weights
 // are rounded to 1/256 like the hardware sampler's, and 8-bit channels are filtered in integers.
 constexpr int kFloatWeights = 8;
+// Step-budget flag of the template argument: the launch's params are bounded (MM_MARCH_BOUNDED) with
+// no counted-step limit (max_steps < 0) and both coarse and fine budgets set (>= 0), as in the
+// scenario preset. The loop's step checks then compile to the two budget tests alone; the other
+// cases run the general checks. Fewer live uniform values leave the kernels without SGPR spills.
+constexpr int kBoundedSteps = 16;
 template <int F> constexpr int kSet = F & 7;
 template <int F> constexpr int kQ8 = (F & kFloatWeights) ? 0 : 1;
+template <int F> constexpr bool kBS = (F & kBoundedSteps) != 0;
 
 using mm::MarchConsts;
 using mm::MarchTextures;
@@ -902,9 +908,13 @@ __device__ __forceinline__ TripSample trip_eval(const MarchConsts& c, const Marc
 // transmittance recurrence, never the loop's control flow.
 // The checks that end a trip before it samples: the hard bound on loop trips (termination even if
 // t += stepSize stops advancing t) and, in bounded mode, the coarse / fine step budgets.
+template <int F>
 __device__ __forceinline__ bool trip_begin(const MarchConsts& c, MarchState& m) {
     if (++m.iterations > MM_MAX_MARCH_ITERATIONS) { m.active = false; return false; }
-    if (c.flags & MM_MARCH_BOUNDED) {
+    if constexpr (kBS<F>) {
+        if (m.noHits) { if (++m.mc0 > c.maxCoarse) { m.active = false; return false; } }
+        else { if (++m.mc1 > c.maxFine) { m.active = false; return false; } }
+    } else if (c.flags & MM_MARCH_BOUNDED) {
         if (m.noHits) { if (c.maxCoarse >= 0 && ++m.mc0 > c.maxCoarse) { m.active = false; return false; } }
         else { if (c.maxFine >= 0 && ++m.mc1 > c.maxFine) { m.active = false; return false; } }
     }
@@ -914,11 +924,12 @@ __device__ __forceinline__ bool trip_begin(const MarchConsts& c, MarchState& m) 
 // After trip_begin: the state update of the trip with its samples (low-res density lo, eroded
 // density dens). A job's mix weight 1 - accum is returned in jobW; the caller takes its in-scatter
 // factor from the trip's own samples.
+template <int F>
 __device__ __forceinline__ bool trip_commit(const MarchConsts& c, const RaySetup& r, MarchState& m, float lo,
                                             float dens, float& jobW) {
     bool job = false;
     const bool coarse = m.noHits;
-    const bool bounded = (c.flags & MM_MARCH_BOUNDED) != 0;
+    const bool bounded = kBS<F> || (c.flags & MM_MARCH_BOUNDED) != 0;
     bool cont = false;  // `continue`: skip the loop tail but still run the increment
     if (lo > 0.0f) {
         m.misses = 0;
@@ -945,7 +956,7 @@ __device__ __forceinline__ bool trip_commit(const MarchConsts& c, const RaySetup
     }
     if (!cont) {
         if (m.accum > c.opacityCutoff) { m.accum = 1.0f; m.active = false; return job; }
-        if (c.maxSteps >= 0 && ++m.steps > c.maxSteps) { m.active = false; return job; }
+        if (!kBS<F> && c.maxSteps >= 0 && ++m.steps > c.maxSteps) { m.active = false; return job; }
         if (!bounded) {
             if (coarse) { if (c.maxCoarse >= 0 && ++m.mc0 > c.maxCoarse) { m.active = false; return job; } }
             else { if (c.maxFine >= 0 && ++m.mc1 > c.maxFine) { m.active = false; return job; } }
@@ -962,10 +973,10 @@ __device__ __forceinline__ bool march_trip(const MarchConsts& c, const MarchText
                                            MarchState& m, V3& jobPos, float& jobStep, float& jobX0, float& jobW,
                                            float& jobRh, float& jobLo, float& jobDens, Weather& jobWeather, Tally& nd,
                                            int& nh) {
-    if (!trip_begin(c, m)) return false;
+    if (!trip_begin<FSET>(c, m)) return false;
     const TripSample s = trip_eval<FSET, COUNT, KEEPW>(c, tx, r, m.noHits, m.stepSize, m.t);
     const float step = m.stepSize;
-    const bool job = trip_commit(c, r, m, s.lo, s.dens, jobW);
+    const bool job = trip_commit<FSET>(c, r, m, s.lo, s.dens, jobW);
     if (job) {
         jobX0 = inscatter_hg(s.lo, s.rHeight, r.HG);
         jobPos = s.pos;
@@ -1322,9 +1333,9 @@ __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchT
             for (int j = 0; j < LPR; j++) {
                 if (!m.active) break;
                 if (j > 0 && !(m.noHits == noHits0 && m.stepSize == step0 && m.t == tj)) break;
-                if (!trip_begin(c, m)) break;
+                if (!trip_begin<FSET>(c, m)) break;
                 float w = 0.0f;
-                const bool job = trip_commit(c, r, m, lo[j], dens[j], w);
+                const bool job = trip_commit<FSET>(c, r, m, lo[j], dens[j], w);
                 if (j == k) {
                     if (COUNT) { nd += s.nd; nh += s.nh; }
                     mine = job;
@@ -1533,10 +1544,13 @@ const void* views_kernel_ptr(int mode, int lanes) {
 }
 
 template <int FSET>
-const void* views_kernel_ptr_c(int mode, int lanes, bool count, bool floatWeights) {
+const void* views_kernel_ptr_c(int mode, int lanes, bool count, bool floatWeights, bool boundedSteps) {
     if (floatWeights)
         return count ? views_kernel_ptr<FSET | kFloatWeights, true>(mode, lanes)
                      : views_kernel_ptr<FSET | kFloatWeights, false>(mode, lanes);
+    if (boundedSteps)
+        return count ? views_kernel_ptr<FSET | kBoundedSteps, true>(mode, lanes)
+                     : views_kernel_ptr<FSET | kBoundedSteps, false>(mode, lanes);
     return count ? views_kernel_ptr<FSET, true>(mode, lanes) : views_kernel_ptr<FSET, false>(mode, lanes);
 }
 
@@ -1607,13 +1621,14 @@ int launch_shadow(int fset, const ShadowConsts& c, const TexView& weather, const
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 
-int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, const ViewDesc* views, int nViews,
-                 const MarchTextures& t, int total, unsigned long long* counts, void* stream) {
+int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, bool boundedSteps, const ViewDesc* views,
+                 int nViews, const MarchTextures& t, int total, unsigned long long* counts, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     const bool cnt = counts != nullptr;
-    const void* k = fset == FSET_REF ? views_kernel_ptr_c<FSET_REF>(mode, lanesPerRay, cnt, floatWeights)
-                    : fset == FSET_FAST ? views_kernel_ptr_c<FSET_FAST>(mode, lanesPerRay, cnt, floatWeights)
-                                        : views_kernel_ptr_c<FSET_PREC>(mode, lanesPerRay, cnt, floatWeights);
+    const bool bs = boundedSteps;
+    const void* k = fset == FSET_REF ? views_kernel_ptr_c<FSET_REF>(mode, lanesPerRay, cnt, floatWeights, bs)
+                    : fset == FSET_FAST ? views_kernel_ptr_c<FSET_FAST>(mode, lanesPerRay, cnt, floatWeights, bs)
+                                        : views_kernel_ptr_c<FSET_PREC>(mode, lanesPerRay, cnt, floatWeights, bs);
     if (!k) return MM_E_FORMAT;
     const unsigned grid = (unsigned)total;
     void* args[] = {(void*)&views, (void*)&nViews, (void*)&t, (void*)&total, (void*)&counts};
@@ -1633,6 +1648,18 @@ int launch_march(int fset, const MarchConsts& c, const MarchTextures& t, float* 
                                          : launch_float_weights<FSET_FAST | kFloatWeights, false>(c, t, out, counts, s);
             case FSET_PREC: return count ? launch_float_weights<FSET_PREC | kFloatWeights, true>(c, t, out, counts, s)
                                          : launch_float_weights<FSET_PREC | kFloatWeights, false>(c, t, out, counts, s);
+        }
+        return MM_E_FORMAT;
+    }
+    if (bounded_steps(c.flags, c.maxSteps, c.maxCoarse, c.maxFine)) {
+        constexpr int B = kBoundedSteps;
+        switch (fset) {
+            case FSET_REF: return count ? launch_t<FSET_REF | B, true>(c, t, out, counts, s)
+                                        : launch_t<FSET_REF | B, false>(c, t, out, counts, s);
+            case FSET_FAST: return count ? launch_t<FSET_FAST | B, true>(c, t, out, counts, s)
+                                         : launch_t<FSET_FAST | B, false>(c, t, out, counts, s);
+            case FSET_PREC: return count ? launch_t<FSET_PREC | B, true>(c, t, out, counts, s)
+                                         : launch_t<FSET_PREC | B, false>(c, t, out, counts, s);
         }
         return MM_E_FORMAT;
     }

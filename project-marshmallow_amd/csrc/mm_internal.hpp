@@ -124,8 +124,14 @@ struct ViewDesc {
     int tileBase;                 // global index of this view's first tile
 };
 
-int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, const ViewDesc* views, int nViews, const MarchTextures& t, int total,
-                 unsigned long long* counts, void* stream);
+// The march's step checks reduce to the two budget tests (the kernels' specialised loop) when the
+// march is bounded with no counted-step limit and both budgets set.
+inline bool bounded_steps(uint32_t flags, int maxSteps, int maxCoarse, int maxFine) {
+    return (flags & MM_MARCH_BOUNDED) != 0 && maxSteps < 0 && maxCoarse >= 0 && maxFine >= 0;
+}
+
+int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, bool boundedSteps, const ViewDesc* views,
+                 int nViews, const MarchTextures& t, int total, unsigned long long* counts, void* stream);
 
 size_t surrogate_model_bytes();
 int surrogate_init(void* model, uint32_t seed, void* stream);
