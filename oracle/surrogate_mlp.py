@@ -10,7 +10,9 @@ exact gradient of that loss, in float64.
 Loss per row (T = exp(-D), p = exp(o0), r = p - T):  r^2 + (softplus(o1) - |r|)^2, with |r| held
 fixed in the second term (the error head regresses the size of the current residual).
 
-``bf16=True`` rounds, like the MFMA kernels, the operands of every matrix product to bfloat16
+``f16_ops=True`` in predict restates the FP16 inference kernels instead (forward_f16).
+
+``bf16=True`` rounds, like the MFMA training kernels, the operands of every matrix product to bfloat16
 (round to nearest even): the weights, the normalised inputs, the hidden activations, and the
 back-propagated dZ; sums and element-wise math stay in float64.
 
@@ -88,9 +90,38 @@ def forward(flat: np.ndarray, xn: np.ndarray, bf16_ops: bool = False):
     return o, zs
 
 
-def predict(flat, stats, rows, bf16_ops: bool = False) -> np.ndarray:
-    """(o0, softplus(o1)) per row, like mm_surrogate_predict."""
-    o, _ = forward(flat, normalise(rows, stats), bf16_ops)
+def f16(x: np.ndarray) -> np.ndarray:
+    """Round to IEEE binary16 (nearest, ties to even), returned as float64."""
+    return np.asarray(x, np.float64).astype(np.float16).astype(np.float64)
+
+
+def hswish_f16(z):
+    """The inference kernels' hard SiLU in packed FP16: v = FP16(z), u = clamp(FP16(fma(v, FP16(1/6), 1/2)),
+    0, 1), FP16(v u). Each FP16 operation is exact in float64 before its one rounding."""
+    v = f16(z)
+    u = np.clip(f16(v * f16(1.0 / 6.0) + 0.5), 0.0, 1.0)
+    return f16(v * u)
+
+
+def forward_f16(flat: np.ndarray, xn: np.ndarray) -> np.ndarray:
+    """Outputs (n, 2) of the FP16 inference path (the march's light-cone batches and
+    mm_surrogate_predict): normalised inputs and weights rounded to FP16, each hidden bias as the
+    sum of its FP16 rounding and the FP16 rounding of the remainder, FP32-accumulated sums (here
+    float64), activations by hswish_f16, output biases added in FP32."""
+    p = unpack(flat)
+    a = f16(xn)
+    for l in (1, 2, 3):
+        b = np.asarray(p[f"b{l}"], np.float32).astype(np.float64)
+        hi = f16(b)
+        z = a @ f16(p[f"W{l}"]) + (hi + f16(b - hi))
+        a = hswish_f16(z)
+    return a @ f16(p["W4"]) + p["b4"]
+
+
+def predict(flat, stats, rows, bf16_ops: bool = False, f16_ops: bool = False) -> np.ndarray:
+    """(o0, softplus(o1)) per row, like mm_surrogate_predict (f16_ops: its FP16 inference rounding)."""
+    xn = normalise(rows, stats)
+    o = forward_f16(flat, xn) if f16_ops else forward(flat, xn, bf16_ops)[0]
     return np.stack([o[:, 0], softplus(o[:, 1])], 1)
 
 

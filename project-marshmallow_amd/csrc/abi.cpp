@@ -439,7 +439,7 @@ static int build_march(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const
         c.surrThreshold = params->surrogate_threshold;
         c.surrParams = mm::surrogate_params(params->surrogate);
         c.surrStats = mm::surrogate_stats(params->surrogate);
-        c.surrFrags = mm::surrogate_frags(params->surrogate);
+        c.surrFrags = mm::surrogate_infer_frags(params->surrogate);
     }
     if (collect) {
         if (fset == mm::FSET_PREC) return MM_E_FORMAT;

@@ -1158,7 +1158,7 @@ __device__ __forceinline__ void run_batch(const MarchConsts& c, const MarchTextu
             cone_features<FSET, COUNT>(c, tx, mk(q.x[lane], q.y[lane], q.z[lane]), q.step[lane], q.rh[lane],
                                        q.lo[lane], q.dens[lane], q.cov[lane], q.type[lane], f, nd);
         for (int k = 0; k < surr::kIn; k++) f[k] = (f[k] - c.surrStats[k]) * c.surrStats[16 + k];
-        const surr::FwdOut o = surr::mlp_forward64(c.surrFrags, c.surrParams, f, nb);
+        const surr::FwdOut o = surr::mlp_infer64_h(c.surrFrags, c.surrParams + surr::oB4, f, nb);
         // the job's share of the pixel: its cone result enters T with weight (1 - accum) * inScatter * HG
         const float bound = lane < nb ? surr::softplus(o.o1) * (q.w[lane] * fabsf(q.x0[lane])) : 0.0f;
         const bool fb = lane < nb && !(bound < c.surrThreshold);

@@ -47,7 +47,7 @@ struct MarchConsts {
     float surrThreshold;
     const float* surrParams;      // FP32 parameters of the surrogate MLP
     const float* surrStats;       // feature mean[16], inverse std[16]
-    const uint4* surrFrags;       // BF16 MFMA weight fragments
+    const uint4* surrFrags;       // FP16 MFMA inference fragments (surr::kHF*)
     int countSlots;               // 4, or 6: counts[4], counts[5] also receive the weather / low-res reads
     float* stateOut;              // lock-step kernel only, or null: per pixel (final opacity accumulator,
                                   // light-energy accumulator), 2 floats at the image's pixel index
@@ -145,6 +145,7 @@ int surrogate_gradient(const void* model, const float* samples, int n, int strid
 const float* surrogate_params(const void* model);
 const float* surrogate_stats(const void* model);
 const uint4* surrogate_frags(const void* model);
+const uint4* surrogate_infer_frags(const void* model);  // FP16 inference fragments (surr::kHF*)
 
 int launch_march(int fset, const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts,
                  void* stream);

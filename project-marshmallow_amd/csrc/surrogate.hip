@@ -17,17 +17,21 @@ namespace {
 // Model block layout (bytes). All regions 256-byte aligned.
 constexpr size_t kPadP = ((kParams * 4 + 255) / 256) * 256;
 constexpr size_t offParams = 0, offGrads = kPadP, offM = 2 * kPadP, offV = 3 * kPadP, offStats = 4 * kPadP,
-                 offFrags = offStats + 512, kModelBytes = offFrags + (size_t)kFrags * 64 * 16;
+                 offFrags = offStats + 512, offHFrags = offFrags + (size_t)kFrags * 64 * 16,
+                 kModelBytes = offHFrags + (size_t)kHFrags * 64 * 16;
+// one packing thread per (fragment, lane): the BF16 training fragments, then the FP16 inference ones
+constexpr int kPackThreads = (kFrags + kHFrags) * 64;
 
 struct ModelPtrs {
     float *params, *grads, *m, *v, *stats;  // stats: mean[16], inv_std[16], step, loss_sum, loss_n
-    uint4* frags;
+    uint4* frags;   // BF16 training fragments (kFrags)
+    uint4* hfrags;  // FP16 inference fragments (kHFrags)
 };
 
 ModelPtrs ptrs(void* base) {
     uint8_t* b = (uint8_t*)base;
     return ModelPtrs{(float*)(b + offParams), (float*)(b + offGrads), (float*)(b + offM), (float*)(b + offV),
-                     (float*)(b + offStats), (uint4*)(b + offFrags)};
+                     (float*)(b + offStats), (uint4*)(b + offFrags), (uint4*)(b + offHFrags)};
 }
 
 __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
@@ -88,10 +92,38 @@ __global__ void stats_final_kernel(const double* acc, int n, float* stats) {
     }
 }
 
-// BF16 A-operand fragments (see surrogate_dev.hpp). One thread per (fragment, lane).
-__global__ void pack_kernel(const float* __restrict__ p, uint4* __restrict__ frags) {
+// FP16 inference fragment element: fragment f, lane half h, column c, element j (k = 8 h + j).
+__device__ float hfrag_value(const float* __restrict__ p, int f, int h, int c, int j) {
+    const int k = 8 * h + j;
+    auto bias = [&](int off, int u) {  // k = 0: FP16 b, k = 1: the FP16 rounding of the remainder
+        const float b = p[off + u], hi = (float)(_Float16)b;
+        return k == 0 ? hi : (k == 1 ? b - hi : 0.0f);
+    };
+    if (f < kHF2) {
+        const int mt = f / 2, u = mt * 32 + c;
+        return (f % 2 == 0) ? bias(oB1, u) : p[oW1 + k * kHidden + u];
+    }
+    if (f < kHF4) {
+        const int base = f < kHF3 ? kHF2 : kHF3, off = f < kHF3 ? oW2 : oW3, boff = f < kHF3 ? oB2 : oB3;
+        const int mt = (f - base) / 5, which = (f - base) % 5, u = mt * 32 + c;
+        return which == 0 ? bias(boff, u) : p[off + kperm(which - 1, h, j) * kHidden + u];
+    }
+    const int kt = f - kHF4;
+    return c < kOut ? p[oW4 + kperm(kt, h, j) * kOut + c] : 0.0f;
+}
+
+// BF16 A-operand fragments (see surrogate_dev.hpp), then the FP16 inference fragments. One thread per
+// (fragment, lane).
+__global__ void pack_kernel(const float* __restrict__ p, uint4* __restrict__ frags, uint4* __restrict__ hfrags) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= kFrags * 64) return;
+    if (t >= kPackThreads) return;
+    if (t >= kFrags * 64) {
+        const int ti = t - kFrags * 64, f = ti / 64, lane = ti % 64;
+        f16x8 r;
+        for (int j = 0; j < 8; j++) r[j] = (_Float16)hfrag_value(p, f, lane >> 5, lane & 31, j);
+        hfrags[ti] = __builtin_bit_cast(uint4, r);
+        return;
+    }
     const int frag = t / 64, lane = t % 64, h = lane >> 5, c = lane & 31;
     float v[8];
     for (int j = 0; j < 8; j++) {
@@ -319,16 +351,16 @@ __global__ void zero_kernel(float* p, int n) {
     if (i < n) p[i] = 0.0f;
 }
 
-// Batch inference: 64 samples per wave; out[i] = (o0, softplus(o1)).
+// Batch inference with the march's FP16 path: 64 samples per wave; out[i] = (o0, softplus(o1)).
 __global__ __launch_bounds__(64) void predict_kernel(const float* __restrict__ x, int n, int stride,
                                                      const float* __restrict__ params, const float* __restrict__ stats,
-                                                     const uint4* __restrict__ frags, float* __restrict__ out) {
+                                                     const uint4* __restrict__ hfrags, float* __restrict__ out) {
     const int lane = threadIdx.x;
     const long i = (long)blockIdx.x * 64 + lane;
     float f[kIn];
     for (int k = 0; k < kIn; k++) f[k] = i < n ? (x[i * stride + k] - stats[k]) * stats[16 + k] : 0.0f;
     const long left = (long)n - (long)blockIdx.x * 64;
-    FwdOut o = mlp_forward64(frags, params, f, left < 64 ? (int)left : 64);
+    FwdOut o = mlp_infer64_h(hfrags, params + oB4, f, left < 64 ? (int)left : 64);
     if (i < n) {
         out[i * 2 + 0] = o.o0;
         out[i * 2 + 1] = softplus(o.o1);
@@ -348,7 +380,7 @@ int surrogate_init(void* model, uint32_t seed, void* stream) {
     ModelPtrs p = ptrs(model);
     hipLaunchKernelGGL(init_kernel, dim3(blocks(kParams, 256)), dim3(256), 0, s, p.params, p.m, p.v, p.stats, seed);
     hipLaunchKernelGGL(zero_kernel, dim3(blocks(kParams, 256)), dim3(256), 0, s, p.grads, kParams);
-    hipLaunchKernelGGL(pack_kernel, dim3(blocks(kFrags * 64, 256)), dim3(256), 0, s, p.params, p.frags);
+    hipLaunchKernelGGL(pack_kernel, dim3(blocks(kPackThreads, 256)), dim3(256), 0, s, p.params, p.frags, p.hfrags);
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 
@@ -380,7 +412,7 @@ int surrogate_train(void* model, const float* samples, int n, int stride, int ba
                            p.grads, lacc, seed, t0 + i, 0, 1.0f / (float)batch);
         hipLaunchKernelGGL(adam_kernel, dim3(blocks(kParams, 256)), dim3(256), 0, s, p.params, p.grads, p.m, p.v, lr,
                            0.9f, 0.999f, 1e-8f, t0 + i + 1);
-        hipLaunchKernelGGL(pack_kernel, dim3(blocks(kFrags * 64, 256)), dim3(256), 0, s, p.params, p.frags);
+        hipLaunchKernelGGL(pack_kernel, dim3(blocks(kPackThreads, 256)), dim3(256), 0, s, p.params, p.frags, p.hfrags);
     }
     int t1 = t0 + steps;
     if (hipMemcpyAsync(step_counter, &t1, sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess) return MM_E_DEVICE;
@@ -395,7 +427,7 @@ int surrogate_load(void* model, const float* params, const float* stats, void* s
         return MM_E_DEVICE;
     if (stats && hipMemcpyAsync(p.stats, stats, sizeof(float) * 2 * kIn, hipMemcpyDeviceToDevice, s) != hipSuccess)
         return MM_E_DEVICE;
-    hipLaunchKernelGGL(pack_kernel, dim3(blocks(kFrags * 64, 256)), dim3(256), 0, s, p.params, p.frags);
+    hipLaunchKernelGGL(pack_kernel, dim3(blocks(kPackThreads, 256)), dim3(256), 0, s, p.params, p.frags, p.hfrags);
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 
@@ -413,12 +445,13 @@ int surrogate_gradient(const void* model, const float* samples, int n, int strid
 int surrogate_predict(const void* model, const float* x, int n, int stride, float* out, void* stream) {
     ModelPtrs p = ptrs(const_cast<void*>(model));
     hipLaunchKernelGGL(predict_kernel, dim3(blocks(n, 64)), dim3(64), 0, (hipStream_t)stream, x, n, stride, p.params,
-                       p.stats, p.frags, out);
+                       p.stats, p.hfrags, out);
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 
 const float* surrogate_params(const void* model) { return ptrs(const_cast<void*>(model)).params; }
 const float* surrogate_stats(const void* model) { return ptrs(const_cast<void*>(model)).stats; }
 const uint4* surrogate_frags(const void* model) { return ptrs(const_cast<void*>(model)).frags; }
+const uint4* surrogate_infer_frags(const void* model) { return ptrs(const_cast<void*>(model)).hfrags; }
 
 }  // namespace mm

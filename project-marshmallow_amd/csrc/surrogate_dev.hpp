@@ -1,6 +1,6 @@
-// Device side of the learned light-cone surrogate: a 16-64-64-64-2 MLP evaluated with
-// gfx950 BF16 MFMA (v_mfma_f32_32x32x16_bf16), FP32 accumulation, shared by the training kernels
-// and by the march's light-cone batches.
+// Device side of the learned light-cone surrogate: a 16-64-64-64-2 MLP on gfx950 MFMA with FP32
+// accumulation. Training runs BF16 (v_mfma_f32_32x32x16_bf16); inference -- the march's light-cone
+// batches and batch prediction -- runs FP16 (v_mfma_f32_32x32x16_f16) with packed-FP16 activations.
 //
 // Layout. The forward pass computes Z^T = W^T X^T per layer: weights are the A operand (M = units),
 // activations the B operand (N = 32 samples per tile). A 32x32 accumulator puts row 8g + 4h + q
@@ -122,97 +122,139 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
 }
 
 
-// Bias of rows crow(r, h) of m-tile `off` as an accumulator initialiser.
-__device__ __forceinline__ f32x16 bias16(const float* __restrict__ p, int off, int h) {
-    f32x16 z;
-    for (int r = 0; r < 16; r++) z[r] = p[off + crow(r, h)];
-    return z;
+__device__ __forceinline__ float softplus(float x) { return x > 20.0f ? x : __logf(1.0f + __expf(x)); }
+
+// ---------------------------------------------------------------- FP16 inference (the march's path)
+// The light-cone batches and mm_surrogate_predict evaluate the trained network with
+// v_mfma_f32_32x32x16_f16 and packed-FP16 activations; training keeps the BF16 kernels above.
+//  * Hidden units: z (FP32 accumulator) rounded to FP16 (round to nearest even, v_cvt_pk_f16_f32),
+//    u = clamp(fma(z, 1/6, 1/2), 0, 1) and z u, each rounded to FP16 (v_pk_fma_f16 with clamp,
+//    v_pk_mul_f16): three instructions per two units, the result already the next layer's B operand.
+//  * Biases: one more MFMA per m-tile, the bias fragment (k = 0: FP16 b, k = 1: FP16 (b - FP16 b))
+//    against a constant B operand that is 1 at k = 0, 1 and 0 elsewhere, so the accumulator starts at
+//    b to about 22 bits with no per-lane vector loads. The two output biases are uniform and added
+//    after the last layer.
+// Inference fragments (one uint4 of 8 FP16 per lane), A-operand layout as the BF16 ones:
+//   L1: m-tile mt at kHF1 + 2 mt: [bias, W1 (k = 16 features)]
+//   L2, L3: m-tile mt at kHF2/kHF3 + 5 mt: [bias, 4 k-steps (k permuted as kperm)]
+//   L4: kHF4 + kt: 4 k-steps (rows 0 and 1 are the two outputs)
+constexpr int kHF1 = 0, kHF2 = 4, kHF3 = 14, kHF4 = 24, kHFrags = 28;  // 28 KiB
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+
+// Fragments through a global-address-space pointer: global loads with a scalar base and 32-bit lane
+// offsets (a generic pointer would give flat loads and 64-bit address arithmetic).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) u32x4 guint4;
+
+__device__ __forceinline__ f16x8 load_hfrag(guint4* __restrict__ frags, int frag, int lane) {
+    const u32x4 u = frags[frag * 64 + lane];
+    return __builtin_bit_cast(f16x8, u);
 }
 
-// Inference forward of one 32-sample tile with a small register footprint: one m-tile at a time,
-// biases folded into the accumulator initialiser. The four weight fragments of an m-tile are loaded
-// together before its MFMA chain, so the chain waits for one load latency rather than four.
-__device__ __forceinline__ FwdOut mlp_infer_tile(const uint4* __restrict__ frags, const float* __restrict__ params,
-                                                 bf16x8 xb) {
-    const int lane = threadIdx.x & 63, h = lane >> 5;
-    bf16x8 b[4], nb[4];
-    for (int mt = 0; mt < 2; mt++) {
-        f32x16 z = bias16(params, oB1 + mt * 32, h);
-        z = mfma(load_frag(frags, kFragF1 + mt, lane), xb, z);
-        float v[16];
-        for (int r = 0; r < 16; r++) v[r] = hswish(z[r]);
-        b[mt * 2] = pack8(v);
-        b[mt * 2 + 1] = pack8(v + 8);
+__device__ __forceinline__ f32x16 mfma_h(f16x8 a, f16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+// Hard SiLU of accumulator elements 8q .. 8q + 7 in packed FP16, as one B-operand k-step.
+__device__ __forceinline__ f16x8 hswish8_h(const f32x16& z, int q) {
+    const h2 k6 = {(_Float16)(1.0f / 6.0f), (_Float16)(1.0f / 6.0f)};
+    const h2 half = {(_Float16)0.5f, (_Float16)0.5f};
+    const h2 zero = {(_Float16)0.0f, (_Float16)0.0f}, one = {(_Float16)1.0f, (_Float16)1.0f};
+    uint32_t w[4];
+    for (int i = 0; i < 4; i++) {
+        const f2 zz = {z[8 * q + 2 * i], z[8 * q + 2 * i + 1]};
+        const h2 v = __builtin_convertvector(zz, h2);
+        h2 u = __builtin_elementwise_fma(v, k6, half);
+        u = __builtin_elementwise_min(__builtin_elementwise_max(u, zero), one);
+        w[i] = __builtin_bit_cast(uint32_t, v * u);
     }
-    const int bias[2] = {oB2, oB3};
-    const int ffrag[2] = {kFragF2, kFragF3};
+    return __builtin_bit_cast(f16x8, make_uint4(w[0], w[1], w[2], w[3]));
+}
+
+// Inference of one 32-sample tile: xb is this lane's B operand of layer 1 (8 normalised FP16 features
+// of sample lane % 32, features 8h .. 8h + 7), ones the constant bias operand. Returns the two
+// outputs of sample lane % 32 in lanes of half 0, without the output biases.
+__device__ __forceinline__ FwdOut mlp_infer_tile_h(guint4* __restrict__ frags, f16x8 xb, f16x8 ones) {
+    const int lane = threadIdx.x & 63;
+    f16x8 b[4];
+    for (int mt = 0; mt < 2; mt++) {
+        const f16x8 wb = load_hfrag(frags, kHF1 + 2 * mt, lane), w = load_hfrag(frags, kHF1 + 2 * mt + 1, lane);
+        f32x16 z = {};
+        z = mfma_h(wb, ones, z);
+        z = mfma_h(w, xb, z);
+        b[2 * mt] = hswish8_h(z, 0);
+        b[2 * mt + 1] = hswish8_h(z, 1);
+    }
 #pragma unroll 1
     for (int layer = 0; layer < 2; layer++) {
-        // the two m-tiles' MFMA chains interleaved: each dependent MFMA has the other chain's in front
-        bf16x8 w0[4], w1[4];
-        for (int kt = 0; kt < 4; kt++) {
-            w0[kt] = load_frag(frags, ffrag[layer] + kt, lane);
-            w1[kt] = load_frag(frags, ffrag[layer] + 4 + kt, lane);
-        }
-        f32x16 zz[2] = {bias16(params, bias[layer], h), bias16(params, bias[layer] + 32, h)};
-        for (int kt = 0; kt < 4; kt++) {
-            zz[0] = mfma(w0[kt], b[kt], zz[0]);
-            zz[1] = mfma(w1[kt], b[kt], zz[1]);
-        }
-        __builtin_amdgcn_sched_barrier(0);
+        const int base = layer ? kHF3 : kHF2;
+        f16x8 nb[4];
         for (int mt = 0; mt < 2; mt++) {
-            const f32x16 z = zz[mt];
-            float v[16];
-            for (int r = 0; r < 16; r++) v[r] = hswish(z[r]);
-            nb[mt * 2] = pack8(v);
-            nb[mt * 2 + 1] = pack8(v + 8);
+            f16x8 w[5];
+            for (int k = 0; k < 5; k++) w[k] = load_hfrag(frags, base + 5 * mt + k, lane);
+            f32x16 z = {};
+            z = mfma_h(w[0], ones, z);
+            for (int kt = 0; kt < 4; kt++) z = mfma_h(w[1 + kt], b[kt], z);
+            nb[2 * mt] = hswish8_h(z, 0);
+            nb[2 * mt + 1] = hswish8_h(z, 1);
         }
         for (int k = 0; k < 4; k++) b[k] = nb[k];
     }
-    bf16x8 w[4];
-    for (int kt = 0; kt < 4; kt++) w[kt] = load_frag(frags, kFragF4 + kt, lane);
+    f16x8 w[4];
+    for (int kt = 0; kt < 4; kt++) w[kt] = load_hfrag(frags, kHF4 + kt, lane);
     f32x16 o = {};
-    for (int kt = 0; kt < 4; kt++) o = mfma(w[kt], b[kt], o);
+    for (int kt = 0; kt < 4; kt++) o = mfma_h(w[kt], b[kt], o);
     __builtin_amdgcn_sched_barrier(0);
     FwdOut r;
-    r.o0 = o[0] + params[oB4 + 0];  // row crow(0, 0) = 0 in lanes of half 0
-    r.o1 = o[1] + params[oB4 + 1];  // row 1
+    r.o0 = o[0];  // row crow(0, 0) = 0 in lanes of half 0
+    r.o1 = o[1];  // row 1
     return r;
 }
 
-// Inference for up to 64 samples (two tiles), one sample per lane. `x` are this lane's normalised
-// features; `n` (wave-uniform) is the number of samples in use, lanes 0..n-1: with n <= 32 the second
-// tile (samples 32..63) is not evaluated and lanes 32..63 return zeros. Every lane of the wave must
-// call it (MFMA is wave-wide). Returns this lane's outputs.
-__device__ __forceinline__ FwdOut mlp_forward64(const uint4* frags, const float* params, const float* x, int n = 64) {
+// FP16 inference for up to 64 samples, one per lane (x: this lane's normalised features; n
+// wave-uniform, lanes 0..n-1 in use; n <= 32 skips the second tile). b4 holds the two output
+// biases. Every lane of the wave must call it. Returns this lane's outputs.
+__device__ __forceinline__ FwdOut mlp_infer64_h(const uint4* frags, const float* b4, const float* x, int n = 64) {
     const int lane = threadIdx.x & 63, h = lane >> 5;
-    // opaque to loop-invariant code motion: keeps the weight loads from being hoisted out of an
+    // opaque to loop-invariant code motion: keeps the fragment loads from being hoisted out of an
     // enclosing loop (where they would hold registers for the whole loop)
     frags = uniform_ptr(frags);
-    params = uniform_ptr(params);
     asm volatile("" : "+s"(frags));
-    asm volatile("" : "+s"(params));
-    const bf16x8 lo = pack8(x), hi = pack8(x + 8);
-    // B operand of tile t: lane (h, c) needs features 8h..8h+7 of sample 32t + c
-    uint4 ul = __builtin_bit_cast(uint4, lo), uh = __builtin_bit_cast(uint4, hi), sl, sh;
-    sl.x = __shfl_xor(ul.x, 32); sl.y = __shfl_xor(ul.y, 32); sl.z = __shfl_xor(ul.z, 32); sl.w = __shfl_xor(ul.w, 32);
-    sh.x = __shfl_xor(uh.x, 32); sh.y = __shfl_xor(uh.y, 32); sh.z = __shfl_xor(uh.z, 32); sh.w = __shfl_xor(uh.w, 32);
-    const bf16x8 xlo = __builtin_bit_cast(bf16x8, sl), xhi = __builtin_bit_cast(bf16x8, sh);
-    const bf16x8 b0 = h ? xhi : lo;  // tile 0 (samples 0..31)
-    const bf16x8 b1 = h ? hi : xlo;  // tile 1 (samples 32..63)
-    FwdOut t0 = mlp_infer_tile(frags, params, b0);
+    guint4* gfrags = (guint4*)frags;
+    uint32_t lw[4], hw[4];
+    for (int i = 0; i < 4; i++) {
+        const f2 a = {x[2 * i], x[2 * i + 1]}, b = {x[8 + 2 * i], x[8 + 2 * i + 1]};
+        lw[i] = __builtin_bit_cast(uint32_t, __builtin_convertvector(a, h2));
+        hw[i] = __builtin_bit_cast(uint32_t, __builtin_convertvector(b, h2));
+    }
+    // B operand of tile t: lane (h, c) needs features 8h .. 8h + 7 of sample 32 t + c
+    uint32_t sl[4], sh[4];
+    for (int i = 0; i < 4; i++) {
+        sl[i] = __shfl_xor(lw[i], 32);
+        sh[i] = __shfl_xor(hw[i], 32);
+    }
+    uint32_t t0[4], t1[4];
+    for (int i = 0; i < 4; i++) {
+        t0[i] = h ? sh[i] : lw[i];  // tile 0 (samples 0..31)
+        t1[i] = h ? hw[i] : sl[i];  // tile 1 (samples 32..63)
+    }
+    const f16x8 b0 = __builtin_bit_cast(f16x8, make_uint4(t0[0], t0[1], t0[2], t0[3]));
+    const f16x8 b1 = __builtin_bit_cast(f16x8, make_uint4(t1[0], t1[1], t1[2], t1[3]));
+    const uint32_t one2 = 0x3c003c00u;  // FP16 1.0 at k = 0 and 1 (lane half 0, elements 0 and 1)
+    const f16x8 ones = __builtin_bit_cast(f16x8, make_uint4(h ? 0u : one2, 0u, 0u, 0u));
+    FwdOut t0o = mlp_infer_tile_h(gfrags, b0, ones);
     __builtin_amdgcn_sched_barrier(0);
-    FwdOut t1;
-    t1.o0 = t1.o1 = 0.0f;
-    if (n > 32) t1 = mlp_infer_tile(frags, params, b1);  // wave-uniform: a half-empty batch skips its second tile
-    // tile 0 results sit in lane c (= sample c); tile 1 results for sample 32 + c sit in lane c
+    FwdOut t1o;
+    t1o.o0 = t1o.o1 = 0.0f;
+    if (n > 32) t1o = mlp_infer_tile_h(gfrags, b1, ones);
+    const float s0 = __shfl_xor(t1o.o0, 32), s1 = __shfl_xor(t1o.o1, 32);
     FwdOut r;
-    const float s0 = __shfl_xor(t1.o0, 32), s1 = __shfl_xor(t1.o1, 32);
-    r.o0 = h ? s0 : t0.o0;
-    r.o1 = h ? s1 : t0.o1;
+    r.o0 = (h ? s0 : t0o.o0) + b4[0];
+    r.o1 = (h ? s1 : t0o.o1) + b4[1];
     return r;
 }
-
-__device__ __forceinline__ float softplus(float x) { return x > 20.0f ? x : __logf(1.0f + __expf(x)); }
 
 }  // namespace surr

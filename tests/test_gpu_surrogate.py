@@ -66,11 +66,12 @@ def test_predict_matches_oracle(mm, gpu):
     m.predict(torch.from_numpy(rows).cuda(), out)
     torch.cuda.synchronize()
     got = out.cpu().numpy().reshape(-1, 2)
-    ref = S.predict(flat, st, rows, bf16_ops=True)
+    # the FP16 inference path (the march's light-cone batches): FP32 MFMA accumulation against float64
+    ref = S.predict(flat, st, rows, f16_ops=True)
     scale = np.max(np.abs(ref), 0)
     err = np.max(np.abs(got - ref), 0) / scale
     print("predict rel err", err, flush=True)
-    assert (err <= 2e-2).all()
+    assert (err <= 5e-3).all()
 
 
 def test_training_fits_a_smooth_target(mm, gpu):

@@ -95,3 +95,38 @@ def test_hard_silu_activation():
     zz = zz[np.abs(np.abs(zz) - 3.0) > 1e-3]
     fd = (S.hswish(zz + 1e-6) - S.hswish(zz - 1e-6)) / 2e-6
     assert np.allclose(S.hswish_grad(zz), fd, atol=1e-6)
+
+
+def test_f16_inference_rounding():
+    """The FP16 inference restatement: binary16 rounding against numpy / torch conversions (ties to
+    even, overflow), the packed hard SiLU within its three roundings of the exact activation, and the
+    whole FP16 forward pass close to the float64 network (a transposed weight or a dropped bias fails)."""
+    import torch
+
+    one = 1.0
+    assert S.f16(np.array([one + 2.0 ** -11]))[0] == 1.0                    # tie -> even
+    assert S.f16(np.array([one + 3 * 2.0 ** -11]))[0] == 1.0 + 2.0 ** -9     # tie -> even
+    assert S.f16(np.array([70000.0]))[0] == np.inf
+    v = np.random.default_rng(3).normal(0, 30, 10000).astype(np.float32)  # FP32 values, as the kernels round
+    assert np.array_equal(S.f16(v), torch.from_numpy(v).half().double().numpy())
+    z = np.linspace(-8, 8, 4001)
+    ex = S.hswish(z)
+    assert np.all(np.abs(S.hswish_f16(z) - ex) <= 3 * 2.0 ** -11 * (np.abs(z) + 0.5) + 2.0 ** -24)
+    assert S.hswish_f16(np.array([-4.0]))[0] == 0.0 and S.hswish_f16(np.array([4.0]))[0] == 4.0
+    rng = np.random.default_rng(11)
+    flat = S.init_params(7)
+    stats = np.concatenate([np.zeros(16), np.ones(16)]).astype(np.float32)
+    rows = rng.uniform(-1, 1, (512, 17)).astype(np.float32)
+    ref = S.predict(flat, stats, rows)
+    got = S.predict(flat, stats, rows, f16_ops=True)
+    scale = np.max(np.abs(ref), 0)
+    good = np.max(np.abs(got - ref))
+    assert np.all(np.max(np.abs(got - ref), 0) <= 5e-3 * scale)
+    p = S.unpack(flat)
+    p["W2"] = p["W2"].T.copy()
+    bad = S.predict(S.pack(p), stats, rows, f16_ops=True)
+    assert np.max(np.abs(bad - ref)) > 0.05 * scale.max()
+    p = S.unpack(flat)
+    p["b2"] = np.zeros_like(p["b2"])
+    bad = S.predict(S.pack(p), stats, rows, f16_ops=True)
+    assert np.max(np.abs(bad - ref)) > 3 * good
