@@ -528,9 +528,24 @@ __device__ __forceinline__ V3 wind_offset(const MarchConsts& c, float h) {
     return (c.windStrength * w) * (c.time + h * 200.0f);
 }
 
+// The light-cone surrogate's density probes only feed the network (the recorder of its training
+// rows takes the same probes), so they use the hardware square root, reciprocal square root,
+// reciprocal, exp2 and log2 (about 1 ulp) without the rounding fix-ups of the march's own samples.
+__device__ __forceinline__ V3 proj_shell_fast(const MarchConsts& c, V3 pt, V3 center) {
+    const V3 d = pt - center;
+    return (c.projRadius * (d * __builtin_amdgcn_rsqf(dot3(d, d)))) + center;
+}
+__device__ __forceinline__ float rel_height_fast(const MarchConsts& c, V3 pt, V3 proj) {
+    const V3 d = pt - proj;
+    return clampf(__builtin_amdgcn_sqrtf(dot3(d, d)) * c.invThickness, 0.0f, 1.0f);
+}
+__device__ __forceinline__ float remapClamped01_fast(float v, float a) {
+    return v > a ? clampf((v - a) * __builtin_amdgcn_rcpf(1.0f - a), 0.0f, 1.0f) : 0.0f;
+}
+
 // wout (optional) receives the weather coverage and type the test read (left untouched when the
-// height skips the read).
-template <int FSET, bool COUNT>
+// height skips the read). EXACT = false: the surrogate probes' arithmetic (proj_shell_fast above).
+template <int FSET, bool COUNT, bool EXACT = true>
 __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTextures& tx, V3 pos, float h, V3 center,
                                             Tally& nd, Weather* wout = nullptr) {
     if (COUNT) nd++;
@@ -540,7 +555,7 @@ __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTex
     if (!(h > 0.0f && h < 0.9f)) return 0.0f;
     if (COUNT) nd += kTallyWeather;
     constexpr int q8 = kQ8<FSET>;
-    V3 proj = proj_shell(c, pos, center);
+    V3 proj = EXACT ? proj_shell(c, pos, center) : proj_shell_fast(c, pos, center);
     Weather w = read_weather<FSET>(tx.weather, c.weatherScale * (proj.x - c.camPos[0]),
                                    c.weatherScale * (proj.z - c.camPos[2]), q8);
     if (wout) *wout = w;
@@ -562,8 +577,9 @@ __device__ __forceinline__ float cloud_test(const MarchConsts& c, const MarchTex
     if (density > 1.0f) return 1.0f;
     // heightBiasCoverage(relativeHeight, min(0.85, cloudInfo.r)) (comp:206-208, 245)
     float e = clampf(remapk(fminf(0.85f, w.coverage), 0.7f, 1.0f / (0.8f - 0.7f), 1.0f, 0.8f - 1.0f), 0.8f, 1.0f);
-    float coverage = pow_rt(h, e);
     float erosion = low_erosion<FSET>(lg, tx.low);
+    if constexpr (!EXACT) return remapClamped01_fast(density, remapClamped01_fast(erosion, powg(h, e)));
+    float coverage = pow_rt(h, e);
     erosion = remapClamped01(erosion, coverage);
     return remapClamped01(density, erosion);
 }
@@ -836,9 +852,9 @@ __device__ __forceinline__ void cone_features(const MarchConsts& c, const MarchT
 #pragma unroll 1
     for (int i = 0; i < 3; i++) {
         V3 lsPos = pos + ((3.0f * stepSize) * ld3(c.probeOffsets[i]));
-        V3 lsProj = proj_shell(c, lsPos, center);
-        float lsHeight = rel_height(c, lsPos, lsProj);
-        f[6 + i] = cloud_test<FSET, COUNT>(c, tx, lsPos + wind_offset(c, lsHeight), lsHeight, center, nd);
+        V3 lsProj = proj_shell_fast(c, lsPos, center);
+        float lsHeight = rel_height_fast(c, lsPos, lsProj);
+        f[6 + i] = cloud_test<FSET, COUNT, false>(c, tx, lsPos + wind_offset(c, lsHeight), lsHeight, center, nd);
     }
     f[9] = stepSize * c.invThickness;
     const float k = 6.283185307179586f * c.hiScale;
