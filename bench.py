@@ -177,6 +177,7 @@ def surrogate_leg(mm, ns, cam, sun, sky, prm, W, H, steps, warmup, stream, train
     for r in (best, every):
         r.pop("params")
     every.pop("counts")
+    surrogate_leg.model = model  # kept for the views leg
     return {"threshold": best["threshold"], "mean_err_255": best["mean_err_255"], "max_err_255": best["max_err_255"],
             "frac_within_1": best["frac_within_1"], "replaced_frac": best["replaced_frac"],
             "within_1_255": best["mean_err_255"] <= 1.0, "march_ms_plain": round(t_plain, 4),
@@ -187,6 +188,44 @@ def surrogate_leg(mm, ns, cam, sun, sky, prm, W, H, steps, warmup, stream, train
             "mfma": {"flop_per_launch": flop, "achieved_tflops": round(flop / (t_surr * 1e-3) / 1e12, 2),
                      "frac_of_bf16_peak": round(flop / (t_surr * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 5)},
             "training": info}
+
+
+def views_leg(mm, model, reps):
+    """The many-view workload: 64 seeded views of log-uniform size from 256x256 to 1920x1080 (camera
+    altitude, sun and coverage per view, one weather map each, shared volumes), marched one dispatch
+    per view like the reference (VulkanApplication.cpp:1061-1071) and as one launch over every view's
+    tiles replayed from a HIP graph, with the plain cone and (given a trained model) with every cone
+    on the surrogate; density samples/s from an instrumented launch; then one autotuner pass over
+    tile shape and coarse step (plain cones)."""
+    import torch
+    from marshmallow_amd import multiview
+
+    specs = multiview.view_specs(n=64, seed=1)
+    vs = multiview.ViewSet(mm, specs)
+    res = {"views": len(specs), "pixels": vs.pixels()}
+    modes = [("plain", mm.march_params("scenario"))]
+    if model is not None:
+        modes.append(("surrogate", mm.march_params("scenario", cone_mode=mm.CONE_SURROGATE, surrogate_threshold=1e30,
+                                                   surrogate=model.ptr)))
+    for name, prm in modes:
+        one = multiview.time_launch(lambda: vs.march_one_by_one(prm), reps=reps)
+        plan = mm.ViewPlan(vs.views(), vs.shared, prm)
+        g = multiview.capture(plan)
+        ms = multiview.time_launch(lambda: g.replay(), reps=reps)
+        cnt = torch.zeros(4, dtype=torch.int64, device="cuda")
+        plan.launch(cnt)
+        torch.cuda.synchronize()
+        plan.close()
+        res[name] = {"ms_graph_launch": round(ms, 3), "ms_one_dispatch_per_view": round(one, 3),
+                     "density_samples": int(cnt[0]), "density_samples_per_s": round(int(cnt[0]) / (ms * 1e-3), 1),
+                     "cone_jobs": int(cnt[2]), "jobs_on_plain_cone": int(cnt[3])}
+    if "surrogate" in res:
+        res["surrogate"]["speedup_vs_plain_graph"] = round(res["plain"]["ms_graph_launch"] /
+                                                           res["surrogate"]["ms_graph_launch"], 3)
+    best, table = multiview.autotune(mm, vs, reps=3)
+    res["autotune_best"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in best.items()}
+    res["autotune_candidates"] = len(table)
+    return res
 
 
 def post_leg(mm, cam, sun, out, W, H, steps, warmup):
@@ -333,6 +372,7 @@ def main():
     ap.add_argument("--no-amortized", action="store_true", help="skip the 1/16-phase frame leg")
     ap.add_argument("--surrogate-steps", type=int, default=20000,
                     help="Adam steps of the learned light-cone surrogate leg (0 = skip the leg)")
+    ap.add_argument("--no-views", action="store_true", help="skip the 64-view graph-launch leg")
     ap.add_argument("--sweep", action="store_true",
                     help="also time the workload's 225-scene sweep (coverage x sun x wind time x weather seed)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -453,6 +493,7 @@ def main():
         "surrogate": None,
         "post": None,
         "amortized_frame": None,
+        "views": None,
         "sweep": None,
     }
     if args.sweep:
@@ -464,6 +505,8 @@ def main():
     if args.surrogate_steps > 0 and not args.no_batch:
         rec["surrogate"] = surrogate_leg(mm, ns, cam, sun, sky, prm, W, H, args.steps, args.warmup, stream,
                                          args.surrogate_steps)
+    if not args.no_views and args.fset == "fast" and not args.no_batch:
+        rec["views"] = views_leg(mm, getattr(surrogate_leg, "model", None), 5)
     if args.cpu_budget > 0:
         cb = cpu_baseline(mm, ns, cam, sun, sky, prm, W, H, args.cpu_budget)
         cb["ms_per_frame_projected"] = round(n_density / cb["value"] * 1e3, 1)

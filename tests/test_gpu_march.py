@@ -346,3 +346,20 @@ def test_six_counter_launch(mm, gpu, tex):
         a, b = [int(v) for v in c4.cpu()], [int(v) for v in c6.cpu()]
         assert b[:4] == a, (lanes, a, b)
         assert b[0] >= b[4] >= b[5] >= b[1] > 0, (lanes, b)
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 8])
+def test_bounded_step_kernels_equal_the_general_loop(mm, gpu, tex, lanes):
+    """The scenario preset (bounded, no counted-step limit) runs the kernels whose step checks are the
+    two budget tests alone; a counted-step limit that never fires (max_steps 10^6) selects the general
+    checks. Both must give the same image and sample counts, full frame and one phase."""
+    W, H = 320, 180
+    cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H, sun_elevation_deg=12.0, time=300.0))
+    ns = upload(mm, tex, "fast")
+    for phase in (-1, 6):
+        a, ca = gpu_march(mm, ns, cam, sun, sky, W, H, phase=phase, counts=True,
+                          params=mm.march_params("scenario", lanes_per_ray=lanes))
+        b, cb = gpu_march(mm, ns, cam, sun, sky, W, H, phase=phase, counts=True,
+                          params=mm.march_params("scenario", lanes_per_ray=lanes, max_steps=1_000_000))
+        assert np.array_equal(a, b), (lanes, phase)
+        assert (ca == cb).all(), (lanes, phase, ca, cb)

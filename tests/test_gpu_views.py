@@ -51,8 +51,9 @@ def test_views_equal_single_view_marches(mm, vs, lanes, tile_width):
 
 def test_views_phase_and_update(mm, vs):
     """A phase plan writes only its phase's pixels. An update that would change the launch shape
-    (phase 5 -> every pixel: more tiles) is refused, and a same-shape update (new suns, same phase)
-    takes effect in a HIP graph captured before it."""
+    (phase 5 -> every pixel: more tiles) or the kernel (bounded-step checks -> the general ones) is
+    refused, and a same-shape update (new suns, same phase) takes effect in a HIP graph captured
+    before it."""
     import torch
 
     from marshmallow_amd import multiview
@@ -75,6 +76,11 @@ def test_views_phase_and_update(mm, vs):
         plan.update(vs.views(), prm)
     assert err.value.code == mm.MM_E_PARAM
     assert plan.info()[0] == t5
+    # params whose step checks need the general loop (a counted-step limit) select another kernel
+    general = mm.march_params("scenario", max_steps=1000)
+    with pytest.raises(mm.MMError) as err:
+        plan.update(vs.views(phase=5), general)
+    assert err.value.code == mm.MM_E_PARAM
 
     g = multiview.capture(plan)
     ubos = [mm.build_ubos(mm.scenario_scene(altitude=s.altitude, aspect=s.width / s.height,
