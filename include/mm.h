@@ -406,7 +406,9 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
                         const mm_march_params* params, mm_view_plan** plan, void* stream);
 /* Replace the views' uniforms / outputs (same count) without reallocating. Synchronises the stream.
  * The launch shape must not change: an update whose views, phases, sizes or params give a different
- * tile layout (any view's tile count) or a different lanes-per-ray returns MM_E_PARAM and leaves
+ * tile layout (any view's tile count), a different lanes-per-ray or a different kernel (params
+ * switching between the bounded-step loop -- MM_MARCH_BOUNDED, max_steps < 0, both budgets >= 0 --
+ * and the general one) returns MM_E_PARAM and leaves
  * the plan as it was, so a graph that captured mm_march_views can be replayed after any accepted
  * update. A different shape needs a new plan (and a new capture). */
 int mm_view_plan_update(mm_view_plan* plan, const mm_view* views, int32_t n_views, const mm_march_params* params,
@@ -420,13 +422,14 @@ void mm_view_plan_destroy(mm_view_plan* plan);
 /* ---------------------------------------------------------------- learned light-cone surrogate
  * No counterpart in the reference: an optional replacement of the six-sample light cone
  * (comp:440-453) by a 16-64-64-64-2 MLP (hard-SiLU hidden units, z * clamp(z / 6 + 1/2, 0, 1))
- * evaluated with BF16 MFMA inside the march. For each
+ * trained with BF16 MFMA and evaluated with FP16 MFMA inside the march. For each
  * cone job it predicts the density along the light D and an error bound e on exp(-D); the job's
  * result enters the pixel's transmittance with weight (1 - accum) * inScatter * HG (comp:464), and
  * jobs whose weighted bound e * (1 - accum) * |inScatter * HG| is >= params.surrogate_threshold
  * take the plain cone, so threshold 0 is the plain march.
  * A model is one device block of mm_surrogate_bytes() bytes (FP32 parameters, Adam state, feature
- * statistics, packed BF16 fragments). Training samples are rows of MM_SURROGATE_ROW floats:
+ * statistics, packed BF16 training and FP16 inference fragments). Training samples are rows of
+ * MM_SURROGATE_ROW floats:
  * 16 features followed by the cone's D, recorded by mm_march_clouds_collect. */
 #define MM_SURROGATE_FEATURES 16
 #define MM_SURROGATE_ROW 17
@@ -442,15 +445,16 @@ int mm_surrogate_fit_stats(void* model, const float* samples, int32_t n, void* s
 int mm_surrogate_train(void* model, const float* samples, int32_t n, int32_t batch, int32_t steps, float lr,
                        uint32_t seed, float* loss_out, void* stream);
 /* Replace the FP32 parameters (device float[MM_SURROGATE_PARAMS]) and/or the feature statistics
- * (device float[32]: mean[16], inverse std[16]) of a model, either may be NULL, and repack its BF16
- * fragments: loading a saved model, or setting weights for a test. */
+ * (device float[32]: mean[16], inverse std[16]) of a model, either may be NULL, and repack its
+ * BF16 and FP16 fragments: loading a saved model, or setting weights for a test. */
 int mm_surrogate_load(void* model, const float* params, const float* stats, void* stream);
 /* Exact mean gradient of the training loss over rows 0..n-1 (no sampling, no update): grad_out =
  * device float[MM_SURROGATE_PARAMS] in the parameter layout, loss_out = device float, the summed
  * per-row loss. For gradient checks against a host implementation. */
 int mm_surrogate_gradient(const void* model, const float* samples, int32_t n, float* grad_out, float* loss_out,
                           void* stream);
-/* Batch inference over n rows of stride MM_SURROGATE_ROW: out[2i] = -D estimate, out[2i+1] = e. */
+/* Batch inference over n rows of stride MM_SURROGATE_ROW with the march's FP16 path: out[2i] = -D
+ * estimate, out[2i+1] = e. */
 int mm_surrogate_predict(const void* model, const float* samples, int32_t n, float* out, void* stream);
 
 /* The march with plain light cones that also records cone jobs as surrogate training rows: a job is
