@@ -278,6 +278,31 @@ int mm_march_clouds_state(const mm_camera_ubo* camera, const mm_sun_ubo* sun, co
                           float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
                           int32_t phase, const mm_march_params* params, float* state_out, void* stream);
 
+/* Tile schedule. No counterpart in the reference, whose dispatch runs its workgroups in launch
+ * order: a frame's time is set as much by when its longest tiles start as by the total work (the
+ * rays that graze the shell near the horizon or cross deep cloud take many times the median's
+ * trips). A schedule records each tile's cost (the most loop trips any of its rays took) in one
+ * launch; the next launch of the same shape (tile count, tiles per row, marched grid, kernel shape)
+ * first orders its tiles longest first from those costs (one workgroup on the stream, bucketed by
+ * cost) and then marches in that order, recording the new costs. Costs are indexed by the tile's
+ * place in the image, so they stay valid as the camera moves; a launch of another shape marches in
+ * the default order and starts a new record. The image does not depend on the order, bit for bit:
+ * the schedule only changes which tiles start first. The lock-step kernel (MM_MARCH_NO_BATCH)
+ * ignores it.
+ * mm_march_schedule_create allocates device buffers for up to max_tiles tiles (a full 1920x1080
+ * frame at the default two lanes per ray has 64800); launches with more tiles return MM_E_SIZE, and
+ * a schedule is used on the device that created it. Launches that share a schedule must be ordered
+ * (one stream): the order buffer is rewritten by each. */
+typedef struct mm_march_schedule mm_march_schedule;
+int mm_march_schedule_create(int32_t max_tiles, mm_march_schedule** schedule);
+void mm_march_schedule_destroy(mm_march_schedule* schedule);
+/* mm_march_clouds with a schedule; counts: NULL or device uint64[4] as for mm_march_clouds_counted. */
+int mm_march_clouds_scheduled(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                              const mm_noise_set* textures, const mm_texture* night_sky,
+                              float* out_rgba32f, int32_t width, int32_t height, int32_t row_pitch_px,
+                              int32_t phase, const mm_march_params* params, mm_march_schedule* schedule,
+                              unsigned long long* counts, void* stream);
+
 /* ---------------------------------------------------------------- temporal reprojection
  * Replaces the dispatch of reproject.comp (VulkanApplication.cpp:1051-1057; the shader body is
  * reproject.comp:91-153): every target pixel is fetched from the previous frame `source` along the

@@ -176,6 +176,12 @@ _SIGS = {
     "mm_view_plan_info": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), _P(ctypes.c_int32)]),
     "mm_march_views": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "mm_view_plan_destroy": (None, [ctypes.c_void_p]),
+    "mm_march_schedule_create": (ctypes.c_int, [ctypes.c_int32, _P(ctypes.c_void_p)]),
+    "mm_march_schedule_destroy": (None, [ctypes.c_void_p]),
+    "mm_march_clouds_scheduled": (ctypes.c_int, [_P(CameraUBO), _P(SunUBO), _P(SkyUBO), _P(NoiseSetC), _P(Texture),
+                                                 ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.c_int32, _P(MarchParams), ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p]),
     "mm_surrogate_bytes": (ctypes.c_size_t, []),
     "mm_surrogate_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
     "mm_surrogate_fit_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
@@ -319,16 +325,48 @@ def current_stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 
 
+class MarchSchedule:
+    """Per-tile cost history for mm_march_clouds_scheduled (longest tiles first from the previous
+    launch of the same shape); holds device buffers for up to ``max_tiles`` tiles."""
+
+    def __init__(self, max_tiles: int = 1 << 18):
+        h = ctypes.c_void_p()
+        check(lib().mm_march_schedule_create(max_tiles, ctypes.byref(h)), "mm_march_schedule_create")
+        self.ptr = h.value
+
+    def close(self) -> None:
+        if self.ptr:
+            lib().mm_march_schedule_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def march_clouds(cam: CameraUBO, sun: SunUBO, sky: SkyUBO, noise: NoiseSet, out, width: int, height: int,
                  phase: int = PHASE_ALL, params: MarchParams | None = None, night: DeviceTexture | None = None,
-                 pitch: int | None = None, stream: int | None = None, counts=None) -> None:
+                 pitch: int | None = None, stream: int | None = None, counts=None,
+                 schedule: MarchSchedule | None = None) -> None:
     """One cloud dispatch into ``out`` (a float32 device tensor of >= height*pitch*4 elements). ``counts``
     (int64 device tensor of 4 or 6 elements) accumulates the work counters of mm_march_clouds_counted
-    (4) or mm_march_clouds_counted_ex (6: also the weather-map and low-res reads)."""
+    (4) or mm_march_clouds_counted_ex (6: also the weather-map and low-res reads). ``schedule``: march
+    through mm_march_clouds_scheduled (4 counters at most)."""
     pitch = width if pitch is None else pitch
     stream = current_stream() if stream is None else stream
     pp = ctypes.byref(params) if params is not None else None
     nt = ctypes.byref(night.desc) if night is not None else None
+    if schedule is not None:
+        if counts is not None and counts.numel() != 4:
+            raise ValueError("a scheduled march takes 4 counters")
+        code = lib().mm_march_clouds_scheduled(ctypes.byref(cam), ctypes.byref(sun), ctypes.byref(sky),
+                                               ctypes.byref(noise.c), nt, out.data_ptr(), width, height, pitch, phase,
+                                               pp, schedule.ptr, counts.data_ptr() if counts is not None else None,
+                                               stream)
+        check(code, "mm_march_clouds_scheduled")
+        return
     if counts is not None and counts.numel() >= 6:
         code = lib().mm_march_clouds_counted_ex(ctypes.byref(cam), ctypes.byref(sun), ctypes.byref(sky),
                                                 ctypes.byref(noise.c), nt, out.data_ptr(), width, height, pitch,

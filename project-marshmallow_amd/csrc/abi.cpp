@@ -499,10 +499,70 @@ static void set_lanes(mm::MarchConsts& c, int lpr) {
     horizon_first_rows(c);
 }
 
+// Per-tile cost history of one launch shape (mm_march_clouds_scheduled).
+struct mm_march_schedule {
+    int capacity = 0;              // tiles the buffers hold
+    unsigned* d_cost = nullptr;    // device: each physical tile's cost in the last launch
+    int* d_order = nullptr;        // device: the order built for the current launch
+    bool history = false;          // d_cost holds the costs of a launch of shape `key`
+    int key[6] = {0, 0, 0, 0, 0, 0};
+    int device = 0;
+};
+
+int mm_march_schedule_create(int32_t max_tiles, mm_march_schedule** schedule) {
+    if (!schedule) return MM_E_NULL;
+    *schedule = nullptr;
+    if (max_tiles < 1 || max_tiles > (1 << 26)) return MM_E_PARAM;
+    mm_march_schedule* s = new (std::nothrow) mm_march_schedule;
+    if (!s) return MM_E_SIZE;
+    if (hipGetDevice(&s->device) != hipSuccess ||
+        hipMalloc((void**)&s->d_cost, sizeof(unsigned) * (size_t)max_tiles) != hipSuccess) {
+        delete s;
+        return MM_E_DEVICE;
+    }
+    if (hipMalloc((void**)&s->d_order, sizeof(int) * (size_t)max_tiles) != hipSuccess) {
+        (void)hipFree(s->d_cost);
+        delete s;
+        return MM_E_DEVICE;
+    }
+    s->capacity = max_tiles;
+    *schedule = s;
+    return MM_OK;
+}
+
+void mm_march_schedule_destroy(mm_march_schedule* schedule) {
+    if (!schedule) return;
+    (void)hipFree(schedule->d_cost);
+    (void)hipFree(schedule->d_order);
+    delete schedule;
+}
+
+// Hooks a schedule into a built launch: the march records each tile's cost, and when the schedule
+// holds the costs of a launch of the same shape, a longest-first order built from them on the stream
+// first. The lock-step kernel takes no schedule (it keeps the shader's structure).
+static int use_schedule(mm::MarchConsts& c, mm_march_schedule* sched, void* stream) {
+    if (!sched || (c.flags & MM_MARCH_NO_BATCH)) return MM_OK;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return MM_E_DEVICE;
+    if (dev != sched->device) return MM_E_PARAM;
+    if (c.tiles > sched->capacity) return MM_E_SIZE;
+    const int key[6] = {c.tiles, c.tilesX, c.gridW, c.gridH, c.lanesPerRay, c.tileShift};
+    if (sched->history && memcmp(key, sched->key, sizeof(key)) == 0) {
+        const int e = mm::launch_tile_order(sched->d_cost, c.tiles, sched->d_order, stream);
+        if (e) return e;
+        c.tileOrder = sched->d_order;
+    }
+    c.tileCost = sched->d_cost;
+    memcpy(sched->key, key, sizeof(key));
+    sched->history = true;
+    return MM_OK;
+}
+
 static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
                       const mm_noise_set* tex, const mm_texture* night_sky, float* out, int32_t width, int32_t height,
                       int32_t pitch, int32_t phase, const mm_march_params* params, unsigned long long* counts,
-                      const CollectArgs* collect, void* stream, float* state = nullptr, int count_slots = 4) {
+                      const CollectArgs* collect, void* stream, float* state = nullptr, int count_slots = 4,
+                      mm_march_schedule* sched = nullptr) {
     mm::MarchConsts c;
     mm::MarchTextures t;
     int fset = 0;
@@ -520,7 +580,19 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
     }
     set_lanes(c, lanes_for(c, params, phase));
     reset_error_state();
+    const int es = use_schedule(c, sched, stream);
+    if (es) return es;
     return mm::launch_march(fset, c, t, out, counts, stream);
+}
+
+int mm_march_clouds_scheduled(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
+                              const mm_noise_set* textures, const mm_texture* night_sky, float* out_rgba32f,
+                              int32_t width, int32_t height, int32_t row_pitch_px, int32_t phase,
+                              const mm_march_params* params, mm_march_schedule* schedule,
+                              unsigned long long* counts, void* stream) {
+    if (!schedule) return MM_E_NULL;
+    return march_impl(camera, sun, sky, textures, night_sky, out_rgba32f, width, height, row_pitch_px, phase, params,
+                      counts, nullptr, stream, nullptr, 4, schedule);
 }
 
 int mm_march_clouds(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,

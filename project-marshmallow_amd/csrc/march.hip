@@ -1030,6 +1030,31 @@ __device__ __forceinline__ void tile_pixel(const MarchConsts& c, int gi, int gj,
     }
 }
 
+// Physical tile index (tile row x tilesX + column) of logical launch tile `logical` (rows in tile_row's
+// horizon-first order).
+__device__ __forceinline__ int phys_tile(const MarchConsts& c, int logical) {
+    return tile_row(c, logical / c.tilesX) * c.tilesX + logical % c.tilesX;
+}
+
+// The physical tile block b marches: from the caller's tile order (a permutation of the physical
+// tiles, longest first, built from the previous launch's costs) or the horizon-first launch order;
+// -1 for a block past the tiles or an out-of-range order entry.
+__device__ __forceinline__ int block_tile(const MarchConsts& c) {
+    const int b = (int)blockIdx.x;
+    if (b >= c.tiles) return -1;
+    const int t = c.tileOrder ? c.tileOrder[b] : phys_tile(c, b);
+    return (unsigned)t < (unsigned)c.tiles ? t : -1;
+}
+
+// The tile's cost for the next launch's order: the most loop trips any of its rays took (a ray's
+// lanes all count its trips). Every lane of the wave takes part.
+__device__ __forceinline__ void record_tile_cost(const MarchConsts& c, int phys, int trips) {
+    if (!c.tileCost) return;  // uniform
+    int v = trips;
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    if (threadIdx.x == 0) c.tileCost[phys] = (unsigned)v;
+}
+
 // Adds a lane's density tally to counts[0] and, for a six-slot counter array, its weather and
 // low-res reads to counts[4] and counts[5].
 __device__ __forceinline__ void add_tally(const MarchConsts& c, unsigned long long* counts, Tally nd) {
@@ -1221,8 +1246,9 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
                                            float* __restrict__ out, unsigned long long* __restrict__ counts) {
     __shared__ typename ConeQueueOf<MODE>::type q;
     const int lane = threadIdx.x;
+    // tile: physical tile index (row x tilesX + column)
     const int gi = (tile % c.tilesX) * (1 << c.tileShift) + (lane & ((1 << c.tileShift) - 1));
-    const int gj = tile_row(c, tile / c.tilesX) * (64 >> c.tileShift) + (lane >> c.tileShift);
+    const int gj = (tile / c.tilesX) * (64 >> c.tileShift) + (lane >> c.tileShift);
     const bool valid = gi < c.gridW && gj < c.gridH;
     int px = 0, py = 0;
     if (valid) tile_pixel(c, gi, gj, px, py);
@@ -1272,6 +1298,7 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
         qn -= nb;
     }
     if (valid && r.march) reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
+    record_tile_cost(c, tile, valid ? m.iterations : 0);
     if (COUNT) {
         add_tally(c, counts, nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
@@ -1286,8 +1313,8 @@ template <int FSET, bool COUNT, int MODE>
 __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, const MarchTextures tx,
                                                            float* __restrict__ out,
                                                            unsigned long long* __restrict__ counts) {
-    const int tile = blockIdx.x;
-    if (tile >= c.tiles) return;  // uniform across the block
+    const int tile = block_tile(c);
+    if (tile < 0) return;  // uniform across the block
     march_tile<FSET, COUNT, MODE>(c, tx, tile, out, counts);
 }
 
@@ -1307,8 +1334,9 @@ __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchT
     constexpr int TH = 64 / LPR / 8;  // tile: 8 x TH rays
     __shared__ typename ConeQueueOf<MODE>::type q;
     const int lane = threadIdx.x, ray = lane / LPR, k = lane % LPR, first = lane - k;
+    // tile: physical tile index (row x tilesX + column)
     const int gi = (tile % c.tilesX) * 8 + (ray & 7);
-    const int gj = tile_row(c, tile / c.tilesX) * TH + (ray >> 3);
+    const int gj = (tile / c.tilesX) * TH + (ray >> 3);
     const bool valid = gi < c.gridW && gj < c.gridH;
     int px = 0, py = 0;
     if (valid) tile_pixel(c, gi, gj, px, py);
@@ -1383,6 +1411,7 @@ __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchT
     }
     if (valid && r.march && k == 0)
         reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
+    record_tile_cost(c, tile, valid ? m.iterations : 0);
     if (COUNT) {
         add_tally(c, counts, nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
@@ -1396,8 +1425,8 @@ __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchT
 template <int FSET, bool COUNT, int LPR>
 __global__ __launch_bounds__(64) void march_rp_kernel(const MarchConsts c, const MarchTextures tx,
                                                       float* __restrict__ out, unsigned long long* __restrict__ counts) {
-    const int tile = blockIdx.x;
-    if (tile >= c.tiles) return;  // uniform across the block
+    const int tile = block_tile(c);
+    if (tile < 0) return;  // uniform across the block
     march_rp_tile<FSET, COUNT, LPR>(c, tx, tile, out, counts);
 }
 
@@ -1413,8 +1442,8 @@ template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
 void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
                             unsigned long long* __restrict__ counts) {
-    const int tile = blockIdx.x;
-    if (tile >= c.tiles) return;
+    const int tile = block_tile(c);
+    if (tile < 0) return;
     march_tile<FSET, COUNT, CONE_SURROGATE>(c, tx, tile, out, counts);
 }
 
@@ -1423,8 +1452,8 @@ template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
 void march_rp_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* __restrict__ out,
                                unsigned long long* __restrict__ counts) {
-    const int tile = blockIdx.x;
-    if (tile >= c.tiles) return;
+    const int tile = block_tile(c);
+    if (tile < 0) return;
     march_rp_tile<FSET, COUNT, 2, CONE_SURROGATE>(c, tx, tile, out, counts);
 }
 
@@ -1524,8 +1553,8 @@ __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, 
     if (g - v.tileBase >= v.c.tiles) return;
     MarchTextures tx = tx0;
     tx.weather = v.weather;
-    if constexpr (LPR > 1) march_rp_tile<FSET, COUNT, LPR>(v.c, tx, g - v.tileBase, v.out, counts);
-    else march_tile<FSET, COUNT, MODE>(v.c, tx, g - v.tileBase, v.out, counts);
+    if constexpr (LPR > 1) march_rp_tile<FSET, COUNT, LPR>(v.c, tx, phys_tile(v.c, g - v.tileBase), v.out, counts);
+    else march_tile<FSET, COUNT, MODE>(v.c, tx, phys_tile(v.c, g - v.tileBase), v.out, counts);
 }
 
 template <int FSET, bool COUNT, int MODE, int LPR>
@@ -1615,6 +1644,53 @@ int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned 
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 
+// ---------------------------------------------------------------- tile schedule
+// Longest-first order of the physical tiles from their costs in the previous launch of the same shape.
+// One workgroup: the largest cost, a histogram over 1024 buckets of cost (bucket = cost * 1023 /
+// (max + 1)), the buckets' start positions from the largest down, then each tile takes the next slot
+// of its bucket (an LDS atomic). Every tile gets exactly one slot of [0, n), so the result is a
+// permutation; the order within a bucket is whatever the atomics give (the image does not depend on
+// the order).
+constexpr int kOrderThreads = 1024, kOrderBuckets = 1024;
+
+__global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const unsigned* __restrict__ cost, int n,
+                                                                    int* __restrict__ order) {
+    __shared__ unsigned cnt[kOrderBuckets];
+    __shared__ unsigned part[kOrderThreads / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    unsigned m = 0;
+    for (int i = t; i < n; i += kOrderThreads) m = max(m, cost[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
+    if (lane == 0) part[wave] = m;
+    cnt[t] = 0u;
+    __syncthreads();
+    m = 0;
+    for (int w = 0; w < kOrderThreads / 64; w++) m = max(m, part[w]);
+    const unsigned long long div = (unsigned long long)m + 1ull;
+    for (int i = t; i < n; i += kOrderThreads)
+        atomicAdd(&cnt[(unsigned)(((unsigned long long)cost[i] * (kOrderBuckets - 1)) / div)], 1u);
+    __syncthreads();
+    // exclusive scan from the top bucket: start[b] = tiles in buckets above b. Thread t owns bucket
+    // kOrderBuckets - 1 - t; inclusive wave scan, then the wave totals.
+    const unsigned mine = cnt[kOrderBuckets - 1 - t];
+    unsigned incl = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned v = (unsigned)__shfl_up((int)incl, o);
+        if (lane >= o) incl += v;
+    }
+    __syncthreads();  // every cnt read above is done before cnt is rewritten below
+    if (lane == 63) part[wave] = incl;
+    __syncthreads();
+    unsigned before = 0;
+    for (int w = 0; w < wave; w++) before += part[w];
+    cnt[kOrderBuckets - 1 - t] = before + incl - mine;  // the bucket's first slot
+    __syncthreads();
+    for (int i = t; i < n; i += kOrderThreads) {
+        const unsigned b = (unsigned)(((unsigned long long)cost[i] * (kOrderBuckets - 1)) / div);
+        order[atomicAdd(&cnt[b], 1u)] = i;
+    }
+}
+
 }  // namespace
 
 namespace mm {
@@ -1649,6 +1725,12 @@ int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, bool bo
     const unsigned grid = (unsigned)total;
     void* args[] = {(void*)&views, (void*)&nViews, (void*)&t, (void*)&total, (void*)&counts};
     if (hipLaunchKernel(k, dim3(grid), dim3(64), args, 0, s) != hipSuccess) return MM_E_DEVICE;
+    return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
+}
+
+int launch_tile_order(const unsigned* cost, int n, int* order, void* stream) {
+    if (n <= 0) return MM_OK;
+    hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, (hipStream_t)stream, cost, n, order);
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 

@@ -54,6 +54,10 @@ struct MarchConsts {
     float* collectBuf;            // [collectCap][17]: 16 features + density along the light
     unsigned* collectCount;
     unsigned collectCap, collectKeep, collectSeed;
+    // tile schedule (mm_march_clouds_scheduled), both null otherwise: the physical tile each block
+    // marches (a permutation of 0 .. tiles - 1) and each physical tile's cost for the next launch
+    const int* tileOrder;
+    unsigned* tileCost;
 };
 
 // Device view of one texture.
@@ -149,6 +153,9 @@ const uint4* surrogate_infer_frags(const void* model);  // FP16 inference fragme
 
 int launch_march(int fset, const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts,
                  void* stream);
+// order <- the n physical tiles sorted by cost, largest first (bucketed: 1024 cost buckets), in one
+// workgroup on `stream`.
+int launch_tile_order(const unsigned* cost, int n, int* order, void* stream);
 
 // Noise generation launchers.
 // allow_side = false keeps every generator on `stream` (no second stream is created or used)
