@@ -119,7 +119,7 @@ SURROGATE_FLOP_PER_JOB = 2 * (16 * 64 + 64 * 64 + 64 * 64 + 64 * 2)  # one forwa
 BF16_PEAK_TFLOPS = 2500.0
 
 
-def surrogate_leg(mm, ns, cam, sun, sky, prm, W, H, steps, warmup, stream, train_steps):
+def surrogate_leg(mm, ns, cam, sun, sky, prm, W, H, steps, warmup, stream, train_steps, scheduled=True):
     """Learned light cone at this frame. Training to the config's spec outside the timed region: 2e7
     recorded cone jobs over 64 seeded 1920x1080 views (weather seeds 1-64, sun U[2, 70] deg, coverage
     U[0.2, 0.9]), Adam, batch 8192. The threshold is the largest of a fixed ladder (0 = plain march,
@@ -155,13 +155,16 @@ def surrogate_leg(mm, ns, cam, sun, sky, prm, W, H, steps, warmup, stream, train
             every = dict(row, counts=c)
 
     def timed(p):
-        for _ in range(warmup):
-            mm.march_clouds(cam, sun, sky, ns, surr, W, H, params=p, stream=stream)
+        # plain and surrogate-lit marches alike: with a tile schedule of their own unless the bench runs
+        # unscheduled (the warm-up launches record the first tile costs)
+        sch = None if scheduled is False else mm.MarchSchedule()
+        for _ in range(max(warmup, 1)):
+            mm.march_clouds(cam, sun, sky, ns, surr, W, H, params=p, stream=stream, schedule=sch)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         e0.record()
         for _ in range(steps):
-            mm.march_clouds(cam, sun, sky, ns, surr, W, H, params=p, stream=stream)
+            mm.march_clouds(cam, sun, sky, ns, surr, W, H, params=p, stream=stream, schedule=sch)
         e1.record()
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps
@@ -254,11 +257,22 @@ def post_leg(mm, cam, sun, out, W, H, steps, warmup):
 
 
 def march_kernel_name(args):
-    """The kernel a full-frame plain march launches: lock-step, cone-batched (1 lane per ray) or
-    ray-parallel (2 lanes per ray by default)."""
+    """The kernel a full-frame plain march launches: lock-step, cone-batched (1 lane per ray: the
+    default of a scheduled march) or ray-parallel (2 lanes per ray: the unscheduled default)."""
     if args.no_batch:
         return "march_kernel"
-    return "march_batched_kernel" if args.lanes_per_ray == 1 else "march_rp_kernel"
+    lanes = args.lanes_per_ray or (2 if args.no_schedule else 1)
+    return "march_batched_kernel" if lanes == 1 else "march_rp_kernel"
+
+
+def lanes_and_tiles(args):
+    """(lanes per ray, tile width) the launch runs with (the library's defaults when 0)."""
+    if args.no_batch:
+        return 1, args.tile_width or 8
+    lanes = args.lanes_per_ray or (2 if args.no_schedule else 1)
+    if lanes > 1:
+        return lanes, 8
+    return 1, args.tile_width or (8 if args.no_schedule else 16)
 
 
 def amortized_leg(mm, ns, cam, sun, sky, prm, W, H, warmup):
@@ -318,16 +332,18 @@ def sweep_leg(mm, args, prm, W, H):
         ns.generate(nparams, stream)
         counts = torch.zeros(4, dtype=torch.int64, device="cuda")
         mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream, counts=counts)
-        for _ in range(args.warmup):
+        # one tile schedule per scene, like a renderer's per-view state (the warm-up records its costs)
+        sched = None if args.no_schedule else mm.MarchSchedule()
+        for _ in range(max(args.warmup, 1 if sched else 0)):
             ns.generate(nparams, stream)
-            mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream)
+            mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream, schedule=sched)
         ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
         torch.cuda.synchronize()
         for e in ev:
             e[0].record()
             ns.generate(nparams, stream)
             e[1].record()
-            mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream)
+            mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream, schedule=sched)
             e[2].record()
         torch.cuda.synchronize()
         frame = sum(e[0].elapsed_time(e[2]) for e in ev) / args.steps
@@ -373,6 +389,8 @@ def main():
     ap.add_argument("--surrogate-steps", type=int, default=20000,
                     help="Adam steps of the learned light-cone surrogate leg (0 = skip the leg)")
     ap.add_argument("--no-views", action="store_true", help="skip the 64-view graph-launch leg")
+    ap.add_argument("--no-schedule", action="store_true",
+                    help="march without the tile schedule (launch-order tiles, two lanes per ray by default)")
     ap.add_argument("--sweep", action="store_true",
                     help="also time the workload's 225-scene sweep (coverage x sun x wind time x weather seed)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -418,9 +436,12 @@ def main():
     torch.cuda.synchronize()
     n_density, n_detail, n_jobs, _, n_weather, n_low = (int(v) for v in counts.cpu())
 
-    for _ in range(args.warmup):
+    # the tile schedule: each launch orders its tiles longest first from the previous launch's tile
+    # costs (the warm-up steps record the first ones); the image does not depend on it
+    sched = None if args.no_schedule else mm.MarchSchedule()
+    for _ in range(max(args.warmup, 1 if sched else 0)):
         ns.generate(nparams, stream)
-        mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream)
+        mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream, schedule=sched)
     torch.cuda.synchronize()
 
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
@@ -431,7 +452,7 @@ def main():
         ev[i][0].record()
         ns.generate(nparams, stream)
         ev[i][1].record()
-        mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream)
+        mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm, stream=stream, schedule=sched)
         ev[i][2].record()
     end.record()
     torch.cuda.synchronize()
@@ -443,6 +464,21 @@ def main():
     march_ms_sorted = sorted(e[1].elapsed_time(e[2]) for e in ev)
     median = lambda v: (v[(len(v) - 1) // 2] + v[len(v) // 2]) / 2
     assert torch.isfinite(out).all().item()
+    unscheduled = None
+    if sched is not None:
+        # the same frame's march without the schedule (launch-order tiles, the unscheduled default
+        # shape), for comparison; outside the timed steps
+        prm_u = mm.march_params("scenario", tile_width=args.tile_width, lanes_per_ray=args.lanes_per_ray)
+        for _ in range(2):
+            mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm_u, stream=stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.steps):
+            mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm_u, stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        unscheduled = round(e0.elapsed_time(e1) / args.steps, 4)
 
     b = BYTES_PER_SAMPLE[args.fset]
     alg_bytes = n_density * b["density"] + n_detail * b["detail"] + W * H * 16
@@ -478,10 +514,13 @@ def main():
                                f"FOV, sun 25 deg, coverage 0.5, t = 0, weather seed 1)",
                    "width": W, "height": H, "format_set": args.fset,
                    "kernel": march_kernel_name(args), "coarse_steps_max": 128, "fine_steps_max": 256,
-                   "tile_width": args.tile_width or 8, "lanes_per_ray": 1 if args.no_batch else (args.lanes_per_ray or 2)},
+                   "tile_width": lanes_and_tiles(args)[1], "lanes_per_ray": lanes_and_tiles(args)[0],
+                   "tile_schedule": None if sched is None else
+                   "longest first from the previous step's tile costs (mm_march_clouds_scheduled)"},
         "ms_per_frame": round(ms_step, 4),
         "frames_per_s": round(1000.0 / ms_step, 2),
         "march_ms": round(march_ms, 4),
+        "march_ms_unscheduled": unscheduled,
         "median_ms": {"frame": round(median(step_ms_sorted), 4), "march": round(median(march_ms_sorted), 4)},
         "noise_ms": round(noise_ms, 4),
         "density_samples_per_frame": n_density,
@@ -504,7 +543,7 @@ def main():
         rec["post"] = post_leg(mm, cam, sun, out, W, H, args.steps, args.warmup)
     if args.surrogate_steps > 0 and not args.no_batch:
         rec["surrogate"] = surrogate_leg(mm, ns, cam, sun, sky, prm, W, H, args.steps, args.warmup, stream,
-                                         args.surrogate_steps)
+                                         args.surrogate_steps, not args.no_schedule)
     if not args.no_views and args.fset == "fast" and not args.no_batch:
         rec["views"] = views_leg(mm, getattr(surrogate_leg, "model", None), 5)
     if args.cpu_budget > 0:

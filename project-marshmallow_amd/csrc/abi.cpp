@@ -474,10 +474,12 @@ static int build_march(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const
 // Lanes per ray of a plain march (params.lanes_per_ray; 0 = the default: 8 for a single 1/16 phase,
 // where there are few waves and each lasts as long as its longest ray, else 2); 1 for the other cone
 // modes and the lock-step kernel. The image does not depend on it.
-static int lanes_for(const mm::MarchConsts& c, const mm_march_params* params, int32_t phase) {
+static int lanes_for(const mm::MarchConsts& c, const mm_march_params* params, int32_t phase,
+                     bool scheduled = false) {
     int lpr = params ? params->lanes_per_ray : 0;
+    const bool one_phase = phase >= 0 || phase == MM_PHASE_FROM_UBO;
     if (lpr == 0)
-        lpr = (phase >= 0 || phase == MM_PHASE_FROM_UBO) ? kDefaultPhaseLanesPerRay : kDefaultFrameLanesPerRay;
+        lpr = one_phase ? kDefaultPhaseLanesPerRay : (scheduled ? 1 : kDefaultFrameLanesPerRay);
     if (c.flags & MM_MARCH_NO_BATCH) return 1;
     // the surrogate-lit march runs with one or two lanes per ray; the sample recorder with one
     if (c.coneMode == MM_CONE_SURROGATE) return lpr >= 2 ? 2 : 1;
@@ -578,7 +580,16 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
         c.flags |= MM_MARCH_NO_BATCH;
         c.stateOut = state;
     }
-    set_lanes(c, lanes_for(c, params, phase));
+    const int lanes = lanes_for(c, params, phase, sched != nullptr);
+    if (sched && lanes == 1 && params && params->tile_width == 0 && phase == MM_PHASE_ALL) {
+        // scheduled full frames default to 16 x 4 tiles (with longest-first order the measured best
+        // shape is one lane per ray in 16-wide tiles)
+        c.tileShift = 4;
+        c.tilesX = (c.gridW + 15) >> 4;
+        c.tilesY = (c.gridH + 3) / 4;
+        c.tiles = c.tilesX * c.tilesY;
+    }
+    set_lanes(c, lanes);
     reset_error_state();
     const int es = use_schedule(c, sched, stream);
     if (es) return es;

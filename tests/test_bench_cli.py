@@ -26,9 +26,16 @@ def test_more_than_one_gpu_exits_with_a_message():
 
 def test_kernel_name_follows_the_options():
     b = _bench()
-    args = lambda **kw: type("A", (), dict({"no_batch": False, "lanes_per_ray": 0}, **kw))
-    assert b.march_kernel_name(args()) == "march_rp_kernel"
-    assert b.march_kernel_name(args(lanes_per_ray=1)) == "march_batched_kernel"
+    args = lambda **kw: type("A", (), dict({"no_batch": False, "lanes_per_ray": 0, "tile_width": 0,
+                                            "no_schedule": False}, **kw))
+    # scheduled (the default): one lane per ray in 16 x 4 tiles; unscheduled: two lanes per ray
+    assert b.march_kernel_name(args()) == "march_batched_kernel"
+    assert b.lanes_and_tiles(args()) == (1, 16)
+    assert b.march_kernel_name(args(no_schedule=True)) == "march_rp_kernel"
+    assert b.lanes_and_tiles(args(no_schedule=True)) == (2, 8)
+    assert b.march_kernel_name(args(lanes_per_ray=2)) == "march_rp_kernel"
+    assert b.march_kernel_name(args(lanes_per_ray=1, no_schedule=True)) == "march_batched_kernel"
+    assert b.lanes_and_tiles(args(lanes_per_ray=1, no_schedule=True)) == (1, 8)
     assert b.march_kernel_name(args(no_batch=True)) == "march_kernel"
 
 

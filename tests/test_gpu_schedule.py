@@ -28,16 +28,20 @@ def _march(mm, ns, ubos, W, H, prm, phase=-1, schedule=None):
     return out.cpu().numpy(), cnt.cpu().numpy()
 
 
-@pytest.mark.parametrize("case", ["frame_2lanes", "frame_1lane", "phase_8lanes", "surrogate_threshold0",
-                                  "reference_preset"])
+@pytest.mark.parametrize("case", ["frame_default", "frame_2lanes", "frame_1lane_8x8", "phase_8lanes",
+                                  "surrogate_threshold0", "reference_preset"])
 def test_scheduled_launches_equal_the_plain_march(mm, gpu, case):
+    """Scheduled full frames default to one lane per ray in 16 x 4 tiles (the plain call to two lanes
+    per ray in 8 x 4); every shape gives the same image."""
     W, H = 480, 270
     ns = _noise(mm)
     phase = -1
-    if case == "frame_2lanes":
+    if case == "frame_default":
         prm = mm.march_params("scenario")
-    elif case == "frame_1lane":
-        prm = mm.march_params("scenario", lanes_per_ray=1, tile_width=16)
+    elif case == "frame_2lanes":
+        prm = mm.march_params("scenario", lanes_per_ray=2)
+    elif case == "frame_1lane_8x8":
+        prm = mm.march_params("scenario", lanes_per_ray=1, tile_width=8)
     elif case == "phase_8lanes":
         prm = mm.march_params("scenario")
         phase = 6
@@ -75,8 +79,8 @@ def test_schedule_capacity_and_arguments(mm, gpu):
 
     ns = _noise(mm)
     prm = mm.march_params("scenario")
-    W, H = 320, 180  # 40 x 45 = 1800 tiles of 8 x 4 rays
-    small = mm.MarchSchedule(1000)
+    W, H = 320, 180  # 20 x 45 = 900 tiles of 16 x 4 rays (the scheduled default shape)
+    small = mm.MarchSchedule(500)
     out = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
     with pytest.raises(mm.MMError) as err:
         mm.march_clouds(*_scene(mm, W, H), ns, out, W, H, params=prm, schedule=small)
