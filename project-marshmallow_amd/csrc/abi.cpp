@@ -917,6 +917,11 @@ struct mm_view_plan {
     mm_noise_set shared_set;          // the shared textures as given (curl, low_res, hi_res)
     std::vector<int> tile_base;       // each view's first tile index (the launch's tile layout)
     int device = 0;
+    // tile schedule: every launch records each global tile's cost, and every launch after the first
+    // marches the tiles longest first by the previous launch's costs (the layout never changes)
+    unsigned* d_cost = nullptr;
+    int* d_order = nullptr;
+    mutable bool launched = false;  // set by mm_march_views, which takes the plan as const
 };
 
 // Every view of a plan is marched with the same kernel shape: params.lanes_per_ray (or
@@ -998,7 +1003,15 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
         return MM_E_FORMAT;
     }
     const size_t bytes = sizeof(mm::ViewDesc) * (size_t)n_views;
+    const size_t nt = (size_t)(total > 0 ? total : 1);
     if (hipMalloc((void**)&p->d_views, bytes) != hipSuccess) {
+        delete p;
+        return MM_E_DEVICE;
+    }
+    if (hipMalloc((void**)&p->d_cost, sizeof(unsigned) * nt) != hipSuccess ||
+        hipMalloc((void**)&p->d_order, sizeof(int) * nt) != hipSuccess) {
+        (void)hipFree(p->d_views);
+        (void)hipFree(p->d_cost);
         delete p;
         return MM_E_DEVICE;
     }
@@ -1006,6 +1019,8 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
     if (hipMemcpyAsync(p->d_views, h.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         (void)hipFree(p->d_views);
+        (void)hipFree(p->d_cost);
+        (void)hipFree(p->d_order);
         delete p;
         return MM_E_DEVICE;
     }
@@ -1059,13 +1074,24 @@ int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* s
     if (!plan) return MM_E_NULL;
     if (plan->total_tiles == 0) return MM_OK;
     reset_error_state();
-    return mm::launch_views(plan->fset, plan->mode, plan->lanes, plan->float_weights, plan->bounded_steps,
-                            plan->d_views, plan->n_views, plan->shared, plan->total_tiles, counts, stream);
+    const int* order = nullptr;
+    if (plan->launched) {
+        const int e = mm::launch_tile_order(plan->d_cost, plan->total_tiles, plan->d_order, stream);
+        if (e) return e;
+        order = plan->d_order;
+    }
+    const int e = mm::launch_views(plan->fset, plan->mode, plan->lanes, plan->float_weights, plan->bounded_steps,
+                                   plan->d_views, plan->n_views, plan->shared, plan->total_tiles, counts, order,
+                                   plan->d_cost, stream);
+    if (!e) plan->launched = true;
+    return e;
 }
 
 void mm_view_plan_destroy(mm_view_plan* plan) {
     if (!plan) return;
     (void)hipFree(plan->d_views);
+    (void)hipFree(plan->d_cost);
+    (void)hipFree(plan->d_order);
     delete plan;
 }
 

@@ -1051,11 +1051,11 @@ __device__ __forceinline__ int block_tile(const MarchConsts& c) {
 // -- plus an eighth of the tile's light-cone jobs, which run in 64-job batches. Every lane of the
 // wave takes part. (Measured at 1080p against the trips alone and against each tile's own duration in
 // the previous launch, this order gave the shortest launches.)
-__device__ __forceinline__ void record_tile_cost(const MarchConsts& c, int phys, int trips, int tileJobs) {
-    if (!c.tileCost) return;  // uniform
+__device__ __forceinline__ void record_tile_cost(unsigned* costSlot, int trips, int tileJobs) {
+    if (!costSlot) return;  // uniform
     int v = trips;
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-    if (threadIdx.x == 0) c.tileCost[phys] = (unsigned)v + (unsigned)tileJobs / 8u;
+    if (threadIdx.x == 0) *costSlot = (unsigned)v + (unsigned)tileJobs / 8u;
 }
 
 // Adds a lane's density tally to counts[0] and, for a six-slot counter array, its weather and
@@ -1246,7 +1246,8 @@ __device__ __forceinline__ void queue_shift(Q& q, const int nb, const int rem) {
 // queue and are evaluated 64 at a time (run_batch), each lane folding its results in creation order.
 template <int FSET, bool COUNT, int MODE>
 __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchTextures& tx, const int tile,
-                                           float* __restrict__ out, unsigned long long* __restrict__ counts) {
+                                           float* __restrict__ out, unsigned long long* __restrict__ counts,
+                                           unsigned* costSlot) {
     __shared__ typename ConeQueueOf<MODE>::type q;
     const int lane = threadIdx.x;
     // tile: physical tile index (row x tilesX + column)
@@ -1303,7 +1304,7 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
         qn -= nb;
     }
     if (valid && r.march) reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
-    record_tile_cost(c, tile, valid ? m.iterations + m.mc1 : 0, tileJobs);
+    record_tile_cost(costSlot, valid ? m.iterations + m.mc1 : 0, tileJobs);
     if (COUNT) {
         add_tally(c, counts, nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
@@ -1320,7 +1321,7 @@ __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, 
                                                            unsigned long long* __restrict__ counts) {
     const int tile = block_tile(c);
     if (tile < 0) return;  // uniform across the block
-    march_tile<FSET, COUNT, MODE>(c, tx, tile, out, counts);
+    march_tile<FSET, COUNT, MODE>(c, tx, tile, out, counts, c.tileCost ? c.tileCost + tile : nullptr);
 }
 
 // ---------------------------------------------------------------- ray-parallel march
@@ -1334,7 +1335,8 @@ __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, 
 // each as long as its longest ray). Light-cone jobs go to the wave's queue like the batched kernel's.
 template <int FSET, bool COUNT, int LPR, int MODE = CONE_PLAIN>
 __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchTextures& tx, const int tile,
-                                              float* __restrict__ out, unsigned long long* __restrict__ counts) {
+                                              float* __restrict__ out, unsigned long long* __restrict__ counts,
+                                              unsigned* costSlot) {
     static_assert(LPR == 2 || LPR == 4 || LPR == 8, "lanes per ray");
     constexpr int TH = 64 / LPR / 8;  // tile: 8 x TH rays
     __shared__ typename ConeQueueOf<MODE>::type q;
@@ -1418,7 +1420,7 @@ __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchT
     }
     if (valid && r.march && k == 0)
         reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
-    record_tile_cost(c, tile, valid ? m.iterations + m.mc1 : 0, tileJobs);
+    record_tile_cost(costSlot, valid ? m.iterations + m.mc1 : 0, tileJobs);
     if (COUNT) {
         add_tally(c, counts, nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
@@ -1434,7 +1436,7 @@ __global__ __launch_bounds__(64) void march_rp_kernel(const MarchConsts c, const
                                                       float* __restrict__ out, unsigned long long* __restrict__ counts) {
     const int tile = block_tile(c);
     if (tile < 0) return;  // uniform across the block
-    march_rp_tile<FSET, COUNT, LPR>(c, tx, tile, out, counts);
+    march_rp_tile<FSET, COUNT, LPR>(c, tx, tile, out, counts, c.tileCost ? c.tileCost + tile : nullptr);
 }
 
 #ifndef MM_SURROGATE_WAVES_PER_EU
@@ -1451,7 +1453,7 @@ void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* 
                             unsigned long long* __restrict__ counts) {
     const int tile = block_tile(c);
     if (tile < 0) return;
-    march_tile<FSET, COUNT, CONE_SURROGATE>(c, tx, tile, out, counts);
+    march_tile<FSET, COUNT, CONE_SURROGATE>(c, tx, tile, out, counts, c.tileCost ? c.tileCost + tile : nullptr);
 }
 
 // The surrogate-lit march with two lanes per ray (the plain march's default shape for full frames).
@@ -1461,7 +1463,7 @@ void march_rp_surrogate_kernel(const MarchConsts c, const MarchTextures tx, floa
                                unsigned long long* __restrict__ counts) {
     const int tile = block_tile(c);
     if (tile < 0) return;
-    march_rp_tile<FSET, COUNT, 2, CONE_SURROGATE>(c, tx, tile, out, counts);
+    march_rp_tile<FSET, COUNT, 2, CONE_SURROGATE>(c, tx, tile, out, counts, c.tileCost ? c.tileCost + tile : nullptr);
 }
 
 // ------------------------------------------------------------------ cloud shadows (model.frag)
@@ -1545,9 +1547,14 @@ __global__ __launch_bounds__(256) void cloud_shadow_kernel(const mm::ShadowConst
 // which is the global work queue; no counter or persistent loop is needed.
 template <int FSET, bool COUNT, int MODE, int LPR = 1>
 __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, int nViews, const MarchTextures& tx0,
-                                            int total, unsigned long long* __restrict__ counts) {
-    const int g = blockIdx.x;
-    if (g >= total) return;  // uniform across the block
+                                            int total, unsigned long long* __restrict__ counts,
+                                            const int* __restrict__ order, unsigned* __restrict__ cost) {
+    // global tile g: the plan's longest-first order (from its previous launch's tile costs) or the
+    // concatenated tile lists in launch order
+    const int b = blockIdx.x;
+    if (b >= total) return;  // uniform across the block
+    const int g = order ? order[b] : b;
+    if ((unsigned)g >= (unsigned)total) return;
     int lo = 0, hi = nViews - 1;  // last view whose first tile is <= g
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -1560,22 +1567,25 @@ __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, 
     if (g - v.tileBase >= v.c.tiles) return;
     MarchTextures tx = tx0;
     tx.weather = v.weather;
-    if constexpr (LPR > 1) march_rp_tile<FSET, COUNT, LPR>(v.c, tx, phys_tile(v.c, g - v.tileBase), v.out, counts);
-    else march_tile<FSET, COUNT, MODE>(v.c, tx, phys_tile(v.c, g - v.tileBase), v.out, counts);
+    unsigned* slot = cost ? cost + g : nullptr;
+    if constexpr (LPR > 1) march_rp_tile<FSET, COUNT, LPR>(v.c, tx, phys_tile(v.c, g - v.tileBase), v.out, counts, slot);
+    else march_tile<FSET, COUNT, MODE>(v.c, tx, phys_tile(v.c, g - v.tileBase), v.out, counts, slot);
 }
 
 template <int FSET, bool COUNT, int MODE, int LPR>
 __global__ __launch_bounds__(64)
 void march_views_kernel(const ViewDesc* __restrict__ views, int nViews, const MarchTextures tx, int total,
-                        unsigned long long* __restrict__ counts) {
-    views_group<FSET, COUNT, MODE, LPR>(views, nViews, tx, total, counts);
+                        unsigned long long* __restrict__ counts, const int* __restrict__ order,
+                        unsigned* __restrict__ cost) {
+    views_group<FSET, COUNT, MODE, LPR>(views, nViews, tx, total, counts, order, cost);
 }
 
 template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
 void march_views_surrogate_kernel(const ViewDesc* __restrict__ views, int nViews, const MarchTextures tx, int total,
-                                  unsigned long long* __restrict__ counts) {
-    views_group<FSET, COUNT, CONE_SURROGATE>(views, nViews, tx, total, counts);
+                                  unsigned long long* __restrict__ counts, const int* __restrict__ order,
+                                  unsigned* __restrict__ cost) {
+    views_group<FSET, COUNT, CONE_SURROGATE>(views, nViews, tx, total, counts, order, cost);
 }
 
 template <int FSET, bool COUNT>
@@ -1654,7 +1664,7 @@ int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned 
 // ---------------------------------------------------------------- tile schedule
 // Longest-first order of the physical tiles from their costs in the previous launch of the same shape.
 // One workgroup: the largest cost, a histogram over 1024 buckets of cost (bucket = cost * 1023 /
-// (max + 1)), the buckets' start positions from the largest down, then each tile takes the next slot
+// (max + 1), rounded down), the buckets' start positions from the largest down, then each tile takes the next slot
 // of its bucket (an LDS atomic). Every tile gets exactly one slot of [0, n), so the result is a
 // permutation; the order within a bucket is whatever the atomics give (the image does not depend on
 // the order).
@@ -1662,24 +1672,57 @@ constexpr int kOrderThreads = 1024, kOrderBuckets = 1024;
 
 __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const unsigned* __restrict__ cost, int n,
                                                                     int* __restrict__ order) {
-    __shared__ unsigned cnt[kOrderBuckets];
+    // kCopies interleaved histograms: lane l counts into copy l % kCopies, so the lanes of one LDS
+    // atomic that fall into one bucket (neighbouring tiles have similar costs) queue at most
+    // 64 / kCopies deep instead of 64. Entry (bucket b, copy r) sits at b * kCopies + r.
+    constexpr int kCopies = 8;
+    __shared__ unsigned cnt[kOrderBuckets * kCopies];
     __shared__ unsigned part[kOrderThreads / 64];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, copy = lane % kCopies;
+    // coalesced visits (thread t takes tiles t, t + 1024, ...), kB loads in flight per pass
+    constexpr int kB = 16;
     unsigned m = 0;
-    for (int i = t; i < n; i += kOrderThreads) m = max(m, cost[i]);
+    for (int base = t; base < n; base += kOrderThreads * kB) {
+        unsigned v[kB];
+#pragma unroll
+        for (int k = 0; k < kB; k++) {
+            const int i = base + k * kOrderThreads;
+            v[k] = i < n ? cost[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kB; k++) m = max(m, v[k]);
+    }
     for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
     if (lane == 0) part[wave] = m;
-    cnt[t] = 0u;
+    for (int e = t; e < kOrderBuckets * kCopies; e += kOrderThreads) cnt[e] = 0u;
     __syncthreads();
     m = 0;
     for (int w = 0; w < kOrderThreads / 64; w++) m = max(m, part[w]);
-    const unsigned long long div = (unsigned long long)m + 1ull;
-    for (int i = t; i < n; i += kOrderThreads)
-        atomicAdd(&cnt[(unsigned)(((unsigned long long)cost[i] * (kOrderBuckets - 1)) / div)], 1u);
+    // bucket = c (kOrderBuckets - 1) / (max + 1) in 32.32 fixed point; c <= max keeps it below
+    // kOrderBuckets - 1
+    const unsigned scale = (unsigned)(((unsigned long long)(kOrderBuckets - 1) << 32) / ((unsigned long long)m + 1ull));
+    for (int base = t; base < n; base += kOrderThreads * kB) {
+        unsigned v[kB];
+#pragma unroll
+        for (int k = 0; k < kB; k++) {
+            const int i = base + k * kOrderThreads;
+            v[k] = i < n ? cost[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kB; k++)
+            if (base + k * kOrderThreads < n) atomicAdd(&cnt[__umulhi(v[k], scale) * kCopies + copy], 1u);
+    }
     __syncthreads();
-    // exclusive scan from the top bucket: start[b] = tiles in buckets above b. Thread t owns bucket
-    // kOrderBuckets - 1 - t; inclusive wave scan, then the wave totals.
-    const unsigned mine = cnt[kOrderBuckets - 1 - t];
+    // exclusive scan over (bucket, copy) from the top bucket down: thread t owns the kCopies entries of
+    // bucket kOrderBuckets - 1 - t; their local prefix, an inclusive wave scan of the totals, then the
+    // wave totals.
+    const int b = kOrderBuckets - 1 - t;
+    unsigned loc[kCopies], mine = 0;
+#pragma unroll
+    for (int r = 0; r < kCopies; r++) {
+        loc[r] = mine;
+        mine += cnt[b * kCopies + r];
+    }
     unsigned incl = mine;
     for (int o = 1; o < 64; o <<= 1) {
         const unsigned v = (unsigned)__shfl_up((int)incl, o);
@@ -1690,11 +1733,22 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const unsigne
     __syncthreads();
     unsigned before = 0;
     for (int w = 0; w < wave; w++) before += part[w];
-    cnt[kOrderBuckets - 1 - t] = before + incl - mine;  // the bucket's first slot
+    const unsigned first = before + incl - mine;  // the bucket's first slot
+#pragma unroll
+    for (int r = 0; r < kCopies; r++) cnt[b * kCopies + r] = first + loc[r];
     __syncthreads();
-    for (int i = t; i < n; i += kOrderThreads) {
-        const unsigned b = (unsigned)(((unsigned long long)cost[i] * (kOrderBuckets - 1)) / div);
-        order[atomicAdd(&cnt[b], 1u)] = i;
+    for (int base = t; base < n; base += kOrderThreads * kB) {
+        unsigned v[kB];
+#pragma unroll
+        for (int k = 0; k < kB; k++) {
+            const int i = base + k * kOrderThreads;
+            v[k] = i < n ? cost[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kB; k++) {
+            const int i = base + k * kOrderThreads;
+            if (i < n) order[atomicAdd(&cnt[__umulhi(v[k], scale) * kCopies + copy], 1u)] = i;
+        }
     }
 }
 
@@ -1721,7 +1775,8 @@ int launch_shadow(int fset, const ShadowConsts& c, const TexView& weather, const
 }
 
 int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, bool boundedSteps, const ViewDesc* views,
-                 int nViews, const MarchTextures& t, int total, unsigned long long* counts, void* stream) {
+                 int nViews, const MarchTextures& t, int total, unsigned long long* counts, const int* order,
+                 unsigned* cost, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     const bool cnt = counts != nullptr;
     const bool bs = boundedSteps;
@@ -1730,7 +1785,8 @@ int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, bool bo
                                         : views_kernel_ptr_c<FSET_PREC>(mode, lanesPerRay, cnt, floatWeights, bs);
     if (!k) return MM_E_FORMAT;
     const unsigned grid = (unsigned)total;
-    void* args[] = {(void*)&views, (void*)&nViews, (void*)&t, (void*)&total, (void*)&counts};
+    void* args[] = {(void*)&views, (void*)&nViews, (void*)&t, (void*)&total, (void*)&counts, (void*)&order,
+                    (void*)&cost};
     if (hipLaunchKernel(k, dim3(grid), dim3(64), args, 0, s) != hipSuccess) return MM_E_DEVICE;
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }

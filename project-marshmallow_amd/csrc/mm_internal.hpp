@@ -134,8 +134,11 @@ inline bool bounded_steps(uint32_t flags, int maxSteps, int maxCoarse, int maxFi
     return (flags & MM_MARCH_BOUNDED) != 0 && maxSteps < 0 && maxCoarse >= 0 && maxFine >= 0;
 }
 
+// order: null (launch order) or a permutation of the plan's global tiles; cost: null or one slot per
+// global tile for the next launch's order.
 int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, bool boundedSteps, const ViewDesc* views,
-                 int nViews, const MarchTextures& t, int total, unsigned long long* counts, void* stream);
+                 int nViews, const MarchTextures& t, int total, unsigned long long* counts, const int* order,
+                 unsigned* cost, void* stream);
 
 size_t surrogate_model_bytes();
 int surrogate_init(void* model, uint32_t seed, void* stream);
