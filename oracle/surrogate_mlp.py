@@ -91,8 +91,10 @@ def forward(flat: np.ndarray, xn: np.ndarray, bf16_ops: bool = False):
 
 
 def f16(x: np.ndarray) -> np.ndarray:
-    """Round to IEEE binary16 (nearest, ties to even), returned as float64."""
-    return np.asarray(x, np.float64).astype(np.float16).astype(np.float64)
+    """Round to IEEE binary16 (nearest, ties to even), returned as float64; beyond the FP16 range the
+    result is ±inf, as the hardware's conversion gives."""
+    with np.errstate(over="ignore"):
+        return np.asarray(x, np.float64).astype(np.float16).astype(np.float64)
 
 
 def hswish_f16(z):
