@@ -38,21 +38,28 @@ def load_package():
     return mod
 
 
-def traffic_from_profiles(fset, width, height):
+def _same_kernel(profiled, kernel):
+    """True when no kernel is asked for, or when the profiled instantiation (`name<...>`) is of it."""
+    return kernel is None or (profiled or "").split("<")[0] == kernel
+
+
+def traffic_from_profiles(fset, width, height, kernel=None):
     """HBM bytes per march launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-kernel
-    calibrated), if a summary for this workload is committed under profiles/."""
+    calibrated), if a summary for this workload (and, when named, this march kernel) is committed
+    under profiles/."""
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*pmc*.json"))):
         try:
             with open(path) as f:
                 d = json.load(f)
         except (OSError, ValueError):
             continue
-        if d.get("format_set") == fset and d.get("width") == width and d.get("height") == height:
+        if (d.get("format_set") == fset and d.get("width") == width and d.get("height") == height
+                and _same_kernel(d.get("kernel"), kernel)):
             return d.get("hbm_bytes_per_launch")
     return None
 
 
-def valu_from_profiles(fset, width, height):
+def valu_from_profiles(fset, width, height, kernel=None):
     """VALU position of the march from a committed rocprofv3 --pmc summary of this workload, if present:
     VALU instructions per launch, the issue fraction SQ_INSTS_VALU / (SIMDs x GRBM_GUI_ACTIVE / 4)
     (above 1 where the SIMDs dual-issue), the share of VALU quad-cycles that dual-issued, and the lane
@@ -63,7 +70,8 @@ def valu_from_profiles(fset, width, height):
                 d = json.load(f)
         except (OSError, ValueError):
             continue
-        if d.get("format_set") == fset and d.get("width") == width and d.get("height") == height:
+        if (d.get("format_set") == fset and d.get("width") == width and d.get("height") == height
+                and _same_kernel((d.get("valu") or {}).get("kernel"), kernel)):
             return d.get("valu")
     return None
 
@@ -323,7 +331,7 @@ def sweep_leg(mm, args, prm, W, H):
     out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
     stream = mm.current_stream()
     b = BYTES_PER_SAMPLE[args.fset]
-    vp = valu_from_profiles(args.fset, W, H)
+    vp = valu_from_profiles(args.fset, W, H, march_kernel_name(args))
     rows = []
     for cov, sun_deg, t, seed in itertools.product(*SWEEP.values()):
         cam, sun, sky = mm.build_ubos(mm.scenario_scene(altitude=200.0, pitch_deg=20.0, fov_deg=60.0, aspect=W / H,
@@ -469,6 +477,7 @@ def main():
         # the same frame's march without the schedule (launch-order tiles, the unscheduled default
         # shape), for comparison; outside the timed steps
         prm_u = mm.march_params("scenario", tile_width=args.tile_width, lanes_per_ray=args.lanes_per_ray)
+        prm_u.flags = prm.flags
         for _ in range(2):
             mm.march_clouds(cam, sun, sky, ns, out, W, H, params=prm_u, stream=stream)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -488,14 +497,14 @@ def main():
     tw, tl, _, _ = TEXEL_BYTES[args.fset]
     gathered = n_weather * 4 * tw + n_low * 8 * tl + n_detail * b["detail"] + W * H * 16
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(args.fset, W, H),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(args.fset, W, H, march_kernel_name(args)),
                 "kernel": march_kernel_name(args), "kernel_ms": round(march_ms, 4),
                 "algorithmic_bytes_per_launch": alg_bytes,
                 "bytes_per_density_sample": b["density"], "bytes_per_detail_sample": b["detail"],
                 "gathered": {"bytes_per_launch": gathered, "achieved": round(gathered / (march_ms * 1e-3) / 1e9, 1),
                              "frac": round(gathered / (march_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                              "weather_reads": n_weather, "low_res_reads": n_low},
-                "valu": valu_live(valu_from_profiles(args.fset, W, H), march_ms)}
+                "valu": valu_live(valu_from_profiles(args.fset, W, H, march_kernel_name(args)), march_ms)}
     rec = {
         "metric": METRIC,
         "value": round(n_density / (ms_step * 1e-3), 1),

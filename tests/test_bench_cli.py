@@ -45,3 +45,8 @@ def test_traffic_comes_from_the_committed_summary():
         d = json.load(f)
     assert b.traffic_from_profiles("fast", 1920, 1080) == d["hbm_bytes_per_launch"] > 0
     assert b.traffic_from_profiles("fast", 640, 480) is None
+    # a summary is used only for the kernel it profiled
+    assert b.traffic_from_profiles("fast", 1920, 1080, d["kernel"].split("<")[0]) == d["hbm_bytes_per_launch"]
+    assert b.traffic_from_profiles("fast", 1920, 1080, "march_kernel") is None
+    assert b.valu_from_profiles("fast", 1920, 1080, "march_kernel") is None
+    assert b.valu_from_profiles("fast", 1920, 1080, d["kernel"].split("<")[0])["insts_per_launch"] > 0
