@@ -539,6 +539,12 @@ void mm_march_schedule_destroy(mm_march_schedule* schedule) {
     delete schedule;
 }
 
+// The launch shape a schedule's costs belong to.
+static void schedule_key(const mm::MarchConsts& c, int key[6]) {
+    key[0] = c.tiles; key[1] = c.tilesX; key[2] = c.gridW; key[3] = c.gridH;
+    key[4] = c.lanesPerRay; key[5] = c.tileShift;
+}
+
 // Hooks a schedule into a built launch: the march records each tile's cost, and when the schedule
 // holds the costs of a launch of the same shape, a longest-first order built from them on the stream
 // first. The lock-step kernel takes no schedule (it keeps the shader's structure).
@@ -548,16 +554,22 @@ static int use_schedule(mm::MarchConsts& c, mm_march_schedule* sched, void* stre
     if (hipGetDevice(&dev) != hipSuccess) return MM_E_DEVICE;
     if (dev != sched->device) return MM_E_PARAM;
     if (c.tiles > sched->capacity) return MM_E_SIZE;
-    const int key[6] = {c.tiles, c.tilesX, c.gridW, c.gridH, c.lanesPerRay, c.tileShift};
+    int key[6];
+    schedule_key(c, key);
     if (sched->history && memcmp(key, sched->key, sizeof(key)) == 0) {
         const int e = mm::launch_tile_order(sched->d_cost, c.tiles, sched->d_order, stream);
         if (e) return e;
         c.tileOrder = sched->d_order;
     }
     c.tileCost = sched->d_cost;
-    memcpy(sched->key, key, sizeof(key));
-    sched->history = true;
     return MM_OK;
+}
+
+// After the march's launch: its costs are the history of its shape, if it was enqueued.
+static void commit_schedule(const mm::MarchConsts& c, mm_march_schedule* sched, int launch_error) {
+    if (!sched || !c.tileCost) return;
+    schedule_key(c, sched->key);
+    sched->history = launch_error == MM_OK;
 }
 
 static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
@@ -581,7 +593,7 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
         c.stateOut = state;
     }
     const int lanes = lanes_for(c, params, phase, sched != nullptr);
-    if (sched && lanes == 1 && params && params->tile_width == 0 && phase == MM_PHASE_ALL) {
+    if (sched && lanes == 1 && (!params || params->tile_width == 0) && phase == MM_PHASE_ALL) {
         // scheduled full frames default to 16 x 4 tiles (with longest-first order the measured best
         // shape is one lane per ray in 16-wide tiles)
         c.tileShift = 4;
@@ -593,7 +605,9 @@ static int march_impl(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const 
     reset_error_state();
     const int es = use_schedule(c, sched, stream);
     if (es) return es;
-    return mm::launch_march(fset, c, t, out, counts, stream);
+    const int el = mm::launch_march(fset, c, t, out, counts, stream);
+    commit_schedule(c, sched, el);
+    return el;
 }
 
 int mm_march_clouds_scheduled(const mm_camera_ubo* camera, const mm_sun_ubo* sun, const mm_sky_ubo* sky,
@@ -1016,7 +1030,10 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
         return MM_E_DEVICE;
     }
     hipStream_t s = (hipStream_t)stream;
+    // zero costs until a launch records them (a plan whose first launch is captured into a graph
+    // orders its first eager launch from these)
     if (hipMemcpyAsync(p->d_views, h.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemsetAsync(p->d_cost, 0, sizeof(unsigned) * nt, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         (void)hipFree(p->d_views);
         (void)hipFree(p->d_cost);

@@ -1663,8 +1663,9 @@ int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned 
 
 // ---------------------------------------------------------------- tile schedule
 // Longest-first order of the physical tiles from their costs in the previous launch of the same shape.
-// One workgroup: the largest cost, a histogram over 1024 buckets of cost (bucket = cost * 1023 /
-// (max + 1), rounded down), the buckets' start positions from the largest down, then each tile takes the next slot
+// One workgroup: the largest cost, a histogram over 1024 buckets of cost (bucket = the cost itself
+// when the largest is below 1023, else cost * 1023 / (max + 1) rounded down), the buckets' start
+// positions from the largest down, then each tile takes the next slot
 // of its bucket (an LDS atomic). Every tile gets exactly one slot of [0, n), so the result is a
 // permutation; the order within a bucket is whatever the atomics give (the image does not depend on
 // the order).
@@ -1698,9 +1699,13 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const unsigne
     __syncthreads();
     m = 0;
     for (int w = 0; w < kOrderThreads / 64; w++) m = max(m, part[w]);
-    // bucket = c (kOrderBuckets - 1) / (max + 1) in 32.32 fixed point; c <= max keeps it below
-    // kOrderBuckets - 1
-    const unsigned scale = (unsigned)(((unsigned long long)(kOrderBuckets - 1) << 32) / ((unsigned long long)m + 1ull));
+    // bucket = c (kOrderBuckets - 1) / (max + 1) in 32.32 fixed point for max >= kOrderBuckets - 1,
+    // where the factor is below 2^32 and c <= max keeps the bucket below kOrderBuckets - 1; smaller
+    // costs are their own buckets (the factor would not fit 32 bits)
+    const bool direct = m < (unsigned)(kOrderBuckets - 1);
+    const unsigned scale =
+        direct ? 0u : (unsigned)(((unsigned long long)(kOrderBuckets - 1) << 32) / ((unsigned long long)m + 1ull));
+    auto bucket = [&](unsigned v) { return direct ? v : __umulhi(v, scale); };
     for (int base = t; base < n; base += kOrderThreads * kB) {
         unsigned v[kB];
 #pragma unroll
@@ -1710,7 +1715,7 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const unsigne
         }
 #pragma unroll
         for (int k = 0; k < kB; k++)
-            if (base + k * kOrderThreads < n) atomicAdd(&cnt[__umulhi(v[k], scale) * kCopies + copy], 1u);
+            if (base + k * kOrderThreads < n) atomicAdd(&cnt[bucket(v[k]) * kCopies + copy], 1u);
     }
     __syncthreads();
     // exclusive scan over (bucket, copy) from the top bucket down: thread t owns the kCopies entries of
@@ -1747,7 +1752,7 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const unsigne
 #pragma unroll
         for (int k = 0; k < kB; k++) {
             const int i = base + k * kOrderThreads;
-            if (i < n) order[atomicAdd(&cnt[__umulhi(v[k], scale) * kCopies + copy], 1u)] = i;
+            if (i < n) order[atomicAdd(&cnt[bucket(v[k]) * kCopies + copy], 1u)] = i;
         }
     }
 }

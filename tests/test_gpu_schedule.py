@@ -29,7 +29,7 @@ def _march(mm, ns, ubos, W, H, prm, phase=-1, schedule=None):
 
 
 @pytest.mark.parametrize("case", ["frame_default", "frame_2lanes", "frame_1lane_8x8", "phase_8lanes",
-                                  "surrogate_threshold0", "reference_preset"])
+                                  "surrogate_threshold0", "reference_preset", "no_params"])
 def test_scheduled_launches_equal_the_plain_march(mm, gpu, case):
     """Scheduled full frames default to one lane per ray in 16 x 4 tiles (the plain call to two lanes
     per ray in 8 x 4); every shape gives the same image."""
@@ -48,8 +48,10 @@ def test_scheduled_launches_equal_the_plain_march(mm, gpu, case):
     elif case == "surrogate_threshold0":
         model = mm.Surrogate(seed=3)
         prm = mm.march_params("scenario", cone_mode=mm.CONE_SURROGATE, surrogate_threshold=0.0, surrogate=model.ptr)
-    else:
+    elif case == "reference_preset":
         prm = mm.march_params("reference")
+    else:
+        prm = None  # NULL params: the reference preset, scheduled in 16 x 4 tiles
     sched = mm.MarchSchedule(1 << 16)
     for rep, (sun_deg, t) in enumerate([(25.0, 0.0), (25.0, 0.0), (12.0, 300.0), (12.0, 300.0)]):
         ubos = _scene(mm, W, H, sun_deg, t)
