@@ -107,17 +107,18 @@ def hswish_f16(z):
 
 def forward_f16(flat: np.ndarray, xn: np.ndarray) -> np.ndarray:
     """Outputs (n, 2) of the FP16 inference path (the march's light-cone batches and
-    mm_surrogate_predict): normalised inputs and weights rounded to FP16, each hidden bias as the
-    sum of its FP16 rounding and the FP16 rounding of the remainder, FP32-accumulated sums (here
-    float64), activations by hswish_f16, output biases added in FP32."""
+    mm_surrogate_predict): normalised inputs rounded to FP16, the BF16 weights of the training pass
+    (held in FP16), each hidden bias as the sum of its FP16 rounding and the FP16 rounding of the
+    remainder, FP32-accumulated sums (here float64), activations by hswish_f16, output biases added
+    in FP32."""
     p = unpack(flat)
     a = f16(xn)
     for l in (1, 2, 3):
         b = np.asarray(p[f"b{l}"], np.float32).astype(np.float64)
         hi = f16(b)
-        z = a @ f16(p[f"W{l}"]) + (hi + f16(b - hi))
+        z = a @ f16(bf16(p[f"W{l}"])) + (hi + f16(b - hi))
         a = hswish_f16(z)
-    return a @ f16(p["W4"]) + p["b4"]
+    return a @ f16(bf16(p["W4"])) + p["b4"]
 
 
 def predict(flat, stats, rows, bf16_ops: bool = False, f16_ops: bool = False) -> np.ndarray:

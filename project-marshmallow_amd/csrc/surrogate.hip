@@ -93,6 +93,11 @@ __global__ void stats_final_kernel(const double* acc, int n, float* stats) {
 }
 
 // FP16 inference fragment element: fragment f, lane half h, column c, element j (k = 8 h + j).
+// The inference weights are the training pass's BF16 weights: FP16 holds a BF16 value exactly when
+// its magnitude lies in [2^-17, 65504] (8 significant bits; FP16 subnormals keep 8 bits down to
+// 2^-17), so the FP16 fragments carry the same weights and only the activations gain precision.
+__device__ float bf16_value(float w) { return (float)(__bf16)w; }
+
 __device__ float hfrag_value(const float* __restrict__ p, int f, int h, int c, int j) {
     const int k = 8 * h + j;
     auto bias = [&](int off, int u) {  // k = 0: FP16 b, k = 1: the FP16 rounding of the remainder
@@ -101,15 +106,15 @@ __device__ float hfrag_value(const float* __restrict__ p, int f, int h, int c, i
     };
     if (f < kHF2) {
         const int mt = f / 2, u = mt * 32 + c;
-        return (f % 2 == 0) ? bias(oB1, u) : p[oW1 + k * kHidden + u];
+        return (f % 2 == 0) ? bias(oB1, u) : bf16_value(p[oW1 + k * kHidden + u]);
     }
     if (f < kHF4) {
         const int base = f < kHF3 ? kHF2 : kHF3, off = f < kHF3 ? oW2 : oW3, boff = f < kHF3 ? oB2 : oB3;
         const int mt = (f - base) / 5, which = (f - base) % 5, u = mt * 32 + c;
-        return which == 0 ? bias(boff, u) : p[off + kperm(which - 1, h, j) * kHidden + u];
+        return which == 0 ? bias(boff, u) : bf16_value(p[off + kperm(which - 1, h, j) * kHidden + u]);
     }
     const int kt = f - kHF4;
-    return c < kOut ? p[oW4 + kperm(kt, h, j) * kOut + c] : 0.0f;
+    return c < kOut ? bf16_value(p[oW4 + kperm(kt, h, j) * kOut + c]) : 0.0f;
 }
 
 // BF16 A-operand fragments (see surrogate_dev.hpp), then the FP16 inference fragments. One thread per

@@ -99,8 +99,9 @@ def test_hard_silu_activation():
 
 def test_f16_inference_rounding():
     """The FP16 inference restatement: binary16 rounding against numpy / torch conversions (ties to
-    even, overflow), the packed hard SiLU within its three roundings of the exact activation, and the
-    whole FP16 forward pass close to the float64 network (a transposed weight or a dropped bias fails)."""
+    even, overflow), the packed hard SiLU within its three roundings of the exact activation, FP16
+    holding BF16 weights exactly, and the whole FP16 forward pass close to the float64 network (a
+    transposed weight or a dropped bias fails)."""
     import torch
 
     one = 1.0
@@ -122,6 +123,13 @@ def test_f16_inference_rounding():
     scale = np.max(np.abs(ref), 0)
     good = np.max(np.abs(got - ref))
     assert np.all(np.max(np.abs(got - ref), 0) <= 5e-3 * scale)
+    # FP16 holds the BF16 weights exactly (magnitudes in [2^-17, 65504]): the inference fragments
+    # carry the training pass's weights
+    w = np.concatenate([S.unpack(flat)[k].ravel() for k in ("W1", "W2", "W3", "W4")])
+    w = w[np.abs(w) >= 2.0 ** -17]
+    assert np.array_equal(S.f16(S.bf16(w)), S.bf16(w))
+    ws = np.array([2.0 ** -17 * 1.00390625, 3.0e-5, 0.7, -65280.0])  # 8-bit values across the range
+    assert np.array_equal(S.f16(S.bf16(ws)), S.bf16(ws))
     p = S.unpack(flat)
     p["W2"] = p["W2"].T.copy()
     bad = S.predict(S.pack(p), stats, rows, f16_ops=True)
