@@ -283,14 +283,16 @@ int mm_march_clouds_state(const mm_camera_ubo* camera, const mm_sun_ubo* sun, co
  * rays that graze the shell near the horizon or cross deep cloud take many times the median's
  * trips). A schedule records each tile's cost (the most loop trips any of its rays took) in one
  * launch; the next launch of the same shape (tile count, tiles per row, marched grid, kernel shape)
- * first orders its tiles longest first from those costs (one workgroup on the stream, bucketed by
- * cost) and then marches in that order, recording the new costs. Costs are indexed by the tile's
+ * first orders its tiles longest first from those costs (a counting sort over log-scale cost
+ * buckets, three small launches on the stream) and then marches in that order, recording the new
+ * costs. Costs are indexed by the tile's
  * place in the image, so they stay valid as the camera moves; a launch of another shape marches in
  * the default order and starts a new record. The image does not depend on the order, bit for bit:
  * the schedule only changes which tiles start first. The lock-step kernel (MM_MARCH_NO_BATCH)
  * ignores it.
  * mm_march_schedule_create allocates device buffers for up to max_tiles tiles (a full 1920x1080
- * frame at the default two lanes per ray has 64800); launches with more tiles return MM_E_SIZE, and
+ * frame has 32400 at the scheduled default of one lane per ray in 16 x 4 tiles, 64800 at two lanes
+ * per ray); launches with more tiles return MM_E_SIZE, and
  * a schedule is used on the device that created it. Launches that share a schedule must be ordered
  * (one stream): the order buffer is rewritten by each. */
 typedef struct mm_march_schedule mm_march_schedule;

@@ -504,9 +504,9 @@ static void set_lanes(mm::MarchConsts& c, int lpr) {
 // Per-tile cost history of one launch shape (mm_march_clouds_scheduled).
 struct mm_march_schedule {
     int capacity = 0;              // tiles the buffers hold
-    unsigned* d_cost = nullptr;    // device: each physical tile's cost in the last launch
+    unsigned* d_sched = nullptr;   // device record: order state, then each physical tile's cost in the last launch
     int* d_order = nullptr;        // device: the order built for the current launch
-    bool history = false;          // d_cost holds the costs of a launch of shape `key`
+    bool history = false;          // d_sched holds the costs of a launch of shape `key`
     int key[6] = {0, 0, 0, 0, 0, 0};
     int device = 0;
 };
@@ -517,13 +517,16 @@ int mm_march_schedule_create(int32_t max_tiles, mm_march_schedule** schedule) {
     if (max_tiles < 1 || max_tiles > (1 << 26)) return MM_E_PARAM;
     mm_march_schedule* s = new (std::nothrow) mm_march_schedule;
     if (!s) return MM_E_SIZE;
-    if (hipGetDevice(&s->device) != hipSuccess ||
-        hipMalloc((void**)&s->d_cost, sizeof(unsigned) * (size_t)max_tiles) != hipSuccess) {
+    const size_t record = sizeof(unsigned) * ((size_t)mm::kSchedHeader + (size_t)max_tiles);
+    if (hipGetDevice(&s->device) != hipSuccess || hipMalloc((void**)&s->d_sched, record) != hipSuccess) {
         delete s;
         return MM_E_DEVICE;
     }
-    if (hipMalloc((void**)&s->d_order, sizeof(int) * (size_t)max_tiles) != hipSuccess) {
-        (void)hipFree(s->d_cost);
+    // a clear record (no bucket counts), complete before any stream uses it
+    if (hipMalloc((void**)&s->d_order, sizeof(int) * (size_t)max_tiles) != hipSuccess ||
+        hipMemsetAsync(s->d_sched, 0, record, nullptr) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
+        (void)hipFree(s->d_sched);
+        (void)hipFree(s->d_order);
         delete s;
         return MM_E_DEVICE;
     }
@@ -534,7 +537,7 @@ int mm_march_schedule_create(int32_t max_tiles, mm_march_schedule** schedule) {
 
 void mm_march_schedule_destroy(mm_march_schedule* schedule) {
     if (!schedule) return;
-    (void)hipFree(schedule->d_cost);
+    (void)hipFree(schedule->d_sched);
     (void)hipFree(schedule->d_order);
     delete schedule;
 }
@@ -557,17 +560,20 @@ static int use_schedule(mm::MarchConsts& c, mm_march_schedule* sched, void* stre
     int key[6];
     schedule_key(c, key);
     if (sched->history && memcmp(key, sched->key, sizeof(key)) == 0) {
-        const int e = mm::launch_tile_order(sched->d_cost, c.tiles, sched->d_order, stream);
-        if (e) return e;
+        const int e = mm::launch_tile_order(sched->d_sched, c.tiles, sched->d_order, stream);
+        if (e) {
+            sched->history = false;  // the next launch marches unordered and clears the record's counts
+            return e;
+        }
         c.tileOrder = sched->d_order;
     }
-    c.tileCost = sched->d_cost;
+    c.tileSched = sched->d_sched;
     return MM_OK;
 }
 
 // After the march's launch: its costs are the history of its shape, if it was enqueued.
 static void commit_schedule(const mm::MarchConsts& c, mm_march_schedule* sched, int launch_error) {
-    if (!sched || !c.tileCost) return;
+    if (!sched || !c.tileSched) return;
     schedule_key(c, sched->key);
     sched->history = launch_error == MM_OK;
 }
@@ -933,7 +939,7 @@ struct mm_view_plan {
     int device = 0;
     // tile schedule: every launch records each global tile's cost, and every launch after the first
     // marches the tiles longest first by the previous launch's costs (the layout never changes)
-    unsigned* d_cost = nullptr;
+    unsigned* d_sched = nullptr;  // schedule record: order state, then the global tiles' costs
     int* d_order = nullptr;
     mutable bool launched = false;  // set by mm_march_views, which takes the plan as const
 };
@@ -1022,21 +1028,22 @@ int mm_view_plan_create(const mm_view* views, int32_t n_views, const mm_noise_se
         delete p;
         return MM_E_DEVICE;
     }
-    if (hipMalloc((void**)&p->d_cost, sizeof(unsigned) * nt) != hipSuccess ||
+    const size_t record = sizeof(unsigned) * ((size_t)mm::kSchedHeader + nt);
+    if (hipMalloc((void**)&p->d_sched, record) != hipSuccess ||
         hipMalloc((void**)&p->d_order, sizeof(int) * nt) != hipSuccess) {
         (void)hipFree(p->d_views);
-        (void)hipFree(p->d_cost);
+        (void)hipFree(p->d_sched);
         delete p;
         return MM_E_DEVICE;
     }
     hipStream_t s = (hipStream_t)stream;
-    // zero costs until a launch records them (a plan whose first launch is captured into a graph
-    // orders its first eager launch from these)
+    // a clear record (no counts, zero costs) until a launch records costs: a plan whose first launch
+    // is captured into a graph orders its first eager launch from these
     if (hipMemcpyAsync(p->d_views, h.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemsetAsync(p->d_cost, 0, sizeof(unsigned) * nt, s) != hipSuccess ||
+        hipMemsetAsync(p->d_sched, 0, record, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         (void)hipFree(p->d_views);
-        (void)hipFree(p->d_cost);
+        (void)hipFree(p->d_sched);
         (void)hipFree(p->d_order);
         delete p;
         return MM_E_DEVICE;
@@ -1093,13 +1100,16 @@ int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* s
     reset_error_state();
     const int* order = nullptr;
     if (plan->launched) {
-        const int e = mm::launch_tile_order(plan->d_cost, plan->total_tiles, plan->d_order, stream);
-        if (e) return e;
+        const int e = mm::launch_tile_order(plan->d_sched, plan->total_tiles, plan->d_order, stream);
+        if (e) {
+            plan->launched = false;  // the next launch marches unordered and clears the record's counts
+            return e;
+        }
         order = plan->d_order;
     }
     const int e = mm::launch_views(plan->fset, plan->mode, plan->lanes, plan->float_weights, plan->bounded_steps,
                                    plan->d_views, plan->n_views, plan->shared, plan->total_tiles, counts, order,
-                                   plan->d_cost, stream);
+                                   plan->d_sched, stream);
     if (!e) plan->launched = true;
     return e;
 }
@@ -1107,7 +1117,7 @@ int mm_march_views(const mm_view_plan* plan, unsigned long long* counts, void* s
 void mm_view_plan_destroy(mm_view_plan* plan) {
     if (!plan) return;
     (void)hipFree(plan->d_views);
-    (void)hipFree(plan->d_cost);
+    (void)hipFree(plan->d_sched);
     (void)hipFree(plan->d_order);
     delete plan;
 }

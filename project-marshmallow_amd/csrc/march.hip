@@ -1039,23 +1039,36 @@ __device__ __forceinline__ int phys_tile(const MarchConsts& c, int logical) {
 // The physical tile block b marches: from the caller's tile order (a permutation of the physical
 // tiles, longest first, built from the previous launch's costs) or the horizon-first launch order;
 // -1 for a block past the tiles or an out-of-range order entry.
+// Block 0 of a scheduled launch clears the record's bucket counts for the next order (the order
+// launches before this one are done with them).
+__device__ __forceinline__ void clear_order_counts(unsigned* sched) {
+    for (int i = threadIdx.x; i < mm::kOrderBuckets; i += blockDim.x) sched[mm::kSchedCount + i] = 0u;
+}
+
 __device__ __forceinline__ int block_tile(const MarchConsts& c) {
     const int b = (int)blockIdx.x;
+    if (b == 0 && c.tileSched) clear_order_counts(c.tileSched);
     if (b >= c.tiles) return -1;
     const int t = c.tileOrder ? c.tileOrder[b] : phys_tile(c, b);
     return (unsigned)t < (unsigned)c.tiles ? t : -1;
 }
+
+// Where a tile records its cost: a schedule record (or null) and the tile's slot in it.
+struct CostRec {
+    unsigned* sched;
+    int index;
+};
 
 // The tile's cost for the next launch's order: the longest ray's loop trips plus its fine-mode trips
 // (those also take the detail sample; counted in bounded mode) -- a ray's lanes all count its trips
 // -- plus an eighth of the tile's light-cone jobs, which run in 64-job batches. Every lane of the
 // wave takes part. (Measured at 1080p against the trips alone and against each tile's own duration in
 // the previous launch, this order gave the shortest launches.)
-__device__ __forceinline__ void record_tile_cost(unsigned* costSlot, int trips, int tileJobs) {
-    if (!costSlot) return;  // uniform
+__device__ __forceinline__ void record_tile_cost(CostRec rec, int trips, int tileJobs) {
+    if (!rec.sched) return;  // uniform
     int v = trips;
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-    if (threadIdx.x == 0) *costSlot = (unsigned)v + (unsigned)tileJobs / 8u;
+    if (threadIdx.x == 0) rec.sched[mm::kSchedHeader + rec.index] = (unsigned)v + (unsigned)tileJobs / 8u;
 }
 
 // Adds a lane's density tally to counts[0] and, for a six-slot counter array, its weather and
@@ -1247,7 +1260,7 @@ __device__ __forceinline__ void queue_shift(Q& q, const int nb, const int rem) {
 template <int FSET, bool COUNT, int MODE>
 __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchTextures& tx, const int tile,
                                            float* __restrict__ out, unsigned long long* __restrict__ counts,
-                                           unsigned* costSlot) {
+                                           CostRec costRec) {
     __shared__ typename ConeQueueOf<MODE>::type q;
     const int lane = threadIdx.x;
     // tile: physical tile index (row x tilesX + column)
@@ -1304,7 +1317,7 @@ __device__ __forceinline__ void march_tile(const MarchConsts& c, const MarchText
         qn -= nb;
     }
     if (valid && r.march) reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
-    record_tile_cost(costSlot, valid ? m.iterations + m.mc1 : 0, tileJobs);
+    record_tile_cost(costRec, valid ? m.iterations + m.mc1 : 0, tileJobs);
     if (COUNT) {
         add_tally(c, counts, nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
@@ -1321,7 +1334,7 @@ __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, 
                                                            unsigned long long* __restrict__ counts) {
     const int tile = block_tile(c);
     if (tile < 0) return;  // uniform across the block
-    march_tile<FSET, COUNT, MODE>(c, tx, tile, out, counts, c.tileCost ? c.tileCost + tile : nullptr);
+    march_tile<FSET, COUNT, MODE>(c, tx, tile, out, counts, CostRec{c.tileSched, tile});
 }
 
 // ---------------------------------------------------------------- ray-parallel march
@@ -1336,7 +1349,7 @@ __global__ __launch_bounds__(64) void march_batched_kernel(const MarchConsts c, 
 template <int FSET, bool COUNT, int LPR, int MODE = CONE_PLAIN>
 __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchTextures& tx, const int tile,
                                               float* __restrict__ out, unsigned long long* __restrict__ counts,
-                                              unsigned* costSlot) {
+                                              CostRec costRec) {
     static_assert(LPR == 2 || LPR == 4 || LPR == 8, "lanes per ray");
     constexpr int TH = 64 / LPR / 8;  // tile: 8 x TH rays
     __shared__ typename ConeQueueOf<MODE>::type q;
@@ -1420,7 +1433,7 @@ __device__ __forceinline__ void march_rp_tile(const MarchConsts& c, const MarchT
     }
     if (valid && r.march && k == 0)
         reinterpret_cast<float4*>(out)[(size_t)py * c.pitch + px] = compose(c, r, m.accum, transmittance);
-    record_tile_cost(costSlot, valid ? m.iterations + m.mc1 : 0, tileJobs);
+    record_tile_cost(costRec, valid ? m.iterations + m.mc1 : 0, tileJobs);
     if (COUNT) {
         add_tally(c, counts, nd);
         atomicAdd(&counts[1], (unsigned long long)nh);
@@ -1436,7 +1449,7 @@ __global__ __launch_bounds__(64) void march_rp_kernel(const MarchConsts c, const
                                                       float* __restrict__ out, unsigned long long* __restrict__ counts) {
     const int tile = block_tile(c);
     if (tile < 0) return;  // uniform across the block
-    march_rp_tile<FSET, COUNT, LPR>(c, tx, tile, out, counts, c.tileCost ? c.tileCost + tile : nullptr);
+    march_rp_tile<FSET, COUNT, LPR>(c, tx, tile, out, counts, CostRec{c.tileSched, tile});
 }
 
 #ifndef MM_SURROGATE_WAVES_PER_EU
@@ -1453,7 +1466,7 @@ void march_surrogate_kernel(const MarchConsts c, const MarchTextures tx, float* 
                             unsigned long long* __restrict__ counts) {
     const int tile = block_tile(c);
     if (tile < 0) return;
-    march_tile<FSET, COUNT, CONE_SURROGATE>(c, tx, tile, out, counts, c.tileCost ? c.tileCost + tile : nullptr);
+    march_tile<FSET, COUNT, CONE_SURROGATE>(c, tx, tile, out, counts, CostRec{c.tileSched, tile});
 }
 
 // The surrogate-lit march with two lanes per ray (the plain march's default shape for full frames).
@@ -1463,7 +1476,7 @@ void march_rp_surrogate_kernel(const MarchConsts c, const MarchTextures tx, floa
                                unsigned long long* __restrict__ counts) {
     const int tile = block_tile(c);
     if (tile < 0) return;
-    march_rp_tile<FSET, COUNT, 2, CONE_SURROGATE>(c, tx, tile, out, counts, c.tileCost ? c.tileCost + tile : nullptr);
+    march_rp_tile<FSET, COUNT, 2, CONE_SURROGATE>(c, tx, tile, out, counts, CostRec{c.tileSched, tile});
 }
 
 // ------------------------------------------------------------------ cloud shadows (model.frag)
@@ -1548,10 +1561,11 @@ __global__ __launch_bounds__(256) void cloud_shadow_kernel(const mm::ShadowConst
 template <int FSET, bool COUNT, int MODE, int LPR = 1>
 __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, int nViews, const MarchTextures& tx0,
                                             int total, unsigned long long* __restrict__ counts,
-                                            const int* __restrict__ order, unsigned* __restrict__ cost) {
+                                            const int* __restrict__ order, unsigned* __restrict__ sched) {
     // global tile g: the plan's longest-first order (from its previous launch's tile costs) or the
     // concatenated tile lists in launch order
     const int b = blockIdx.x;
+    if (b == 0 && sched) clear_order_counts(sched);
     if (b >= total) return;  // uniform across the block
     const int g = order ? order[b] : b;
     if ((unsigned)g >= (unsigned)total) return;
@@ -1567,25 +1581,25 @@ __device__ __forceinline__ void views_group(const ViewDesc* __restrict__ views, 
     if (g - v.tileBase >= v.c.tiles) return;
     MarchTextures tx = tx0;
     tx.weather = v.weather;
-    unsigned* slot = cost ? cost + g : nullptr;
-    if constexpr (LPR > 1) march_rp_tile<FSET, COUNT, LPR>(v.c, tx, phys_tile(v.c, g - v.tileBase), v.out, counts, slot);
-    else march_tile<FSET, COUNT, MODE>(v.c, tx, phys_tile(v.c, g - v.tileBase), v.out, counts, slot);
+    const CostRec rec{sched, g};
+    if constexpr (LPR > 1) march_rp_tile<FSET, COUNT, LPR>(v.c, tx, phys_tile(v.c, g - v.tileBase), v.out, counts, rec);
+    else march_tile<FSET, COUNT, MODE>(v.c, tx, phys_tile(v.c, g - v.tileBase), v.out, counts, rec);
 }
 
 template <int FSET, bool COUNT, int MODE, int LPR>
 __global__ __launch_bounds__(64)
 void march_views_kernel(const ViewDesc* __restrict__ views, int nViews, const MarchTextures tx, int total,
                         unsigned long long* __restrict__ counts, const int* __restrict__ order,
-                        unsigned* __restrict__ cost) {
-    views_group<FSET, COUNT, MODE, LPR>(views, nViews, tx, total, counts, order, cost);
+                        unsigned* __restrict__ sched) {
+    views_group<FSET, COUNT, MODE, LPR>(views, nViews, tx, total, counts, order, sched);
 }
 
 template <int FSET, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MM_SURROGATE_WAVES_PER_EU)))
 void march_views_surrogate_kernel(const ViewDesc* __restrict__ views, int nViews, const MarchTextures tx, int total,
                                   unsigned long long* __restrict__ counts, const int* __restrict__ order,
-                                  unsigned* __restrict__ cost) {
-    views_group<FSET, COUNT, CONE_SURROGATE>(views, nViews, tx, total, counts, order, cost);
+                                  unsigned* __restrict__ sched) {
+    views_group<FSET, COUNT, CONE_SURROGATE>(views, nViews, tx, total, counts, order, sched);
 }
 
 template <int FSET, bool COUNT>
@@ -1662,97 +1676,108 @@ int launch_t(const MarchConsts& c, const MarchTextures& t, float* out, unsigned 
 }
 
 // ---------------------------------------------------------------- tile schedule
-// Longest-first order of the physical tiles from their costs in the previous launch of the same shape.
-// One workgroup: the largest cost, a histogram over 1024 buckets of cost (bucket = the cost itself
-// when the largest is below 1023, else cost * 1023 / (max + 1) rounded down), the buckets' start
-// positions from the largest down, then each tile takes the next slot
-// of its bucket (an LDS atomic). Every tile gets exactly one slot of [0, n), so the result is a
-// permutation; the order within a bucket is whatever the atomics give (the image does not depend on
-// the order).
-constexpr int kOrderThreads = 1024, kOrderBuckets = 1024;
+// Longest-first order of the physical tiles from their costs in the previous launch of the same shape,
+// in three launches over blocks of kOrderChunk tiles: (1) each block's histogram of cost buckets
+// added into the record's per-bucket counts; (2) one workgroup turns the counts into each bucket's
+// first slot, from the largest bucket down; (3) each block reserves a run of slots per bucket with one
+// global atomic and hands them to its tiles with LDS atomics. Every tile gets exactly one slot of
+// [0, n), so the result is a permutation; the order within a bucket is whatever the atomics give
+// (the image does not depend on the order). The march that follows clears the counts (its block 0)
+// and records the new costs.
+constexpr int kOrderBlock = 256, kOrderPer = 16, kOrderChunk = kOrderBlock * kOrderPer;
+// The LDS histograms are kept in kOrderCopies interleaved copies (thread t counts into copy t % 8):
+// neighbouring tiles have similar costs, and one LDS atomic whose lanes fall into one bucket then
+// queues 8 deep instead of 64. Entry (bucket b, copy r) sits at b * kOrderCopies + r.
+constexpr int kOrderCopies = 8;
+using mm::kOrderBuckets;
 
-__global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const unsigned* __restrict__ cost, int n,
-                                                                    int* __restrict__ order) {
-    // kCopies interleaved histograms: lane l counts into copy l % kCopies, so the lanes of one LDS
-    // atomic that fall into one bucket (neighbouring tiles have similar costs) queue at most
-    // 64 / kCopies deep instead of 64. Entry (bucket b, copy r) sits at b * kCopies + r.
-    constexpr int kCopies = 8;
-    __shared__ unsigned cnt[kOrderBuckets * kCopies];
-    __shared__ unsigned part[kOrderThreads / 64];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, copy = lane % kCopies;
-    // coalesced visits (thread t takes tiles t, t + 1024, ...), kB loads in flight per pass
-    constexpr int kB = 16;
-    unsigned m = 0;
-    for (int base = t; base < n; base += kOrderThreads * kB) {
-        unsigned v[kB];
+// Cost buckets on a log scale with 32 steps per octave: costs below 32 are their own buckets, a larger
+// cost 2^e (1 + m / 32 + ...) goes to 32 + 32 (e - 5) + m. Monotone in the cost, at most 895, and
+// within 3 % of the cost everywhere; needs no bound on the costs. (Ordering a recorded 1080p frame's
+// 64 800 tile durations by these buckets, a list-scheduling model of the 1536 wave slots reaches the
+// same makespan as the exact longest-first order: the total-work bound.)
+__device__ __forceinline__ unsigned cost_bucket(unsigned c) {
+    if (c < 32u) return c;
+    const unsigned e = 31u - (unsigned)__clz((int)c);  // 5 .. 31
+    return 32u + (e - 5u) * 32u + ((c >> (e - 5u)) & 31u);
+}
+static_assert(32 + 26 * 32 + 31 < mm::kOrderBuckets, "cost buckets");
+
+// This block's tiles' costs (kOrderPer per thread, coalesced) counted per bucket into cnt.
+__device__ __forceinline__ void order_block_counts(const unsigned* __restrict__ cost, int n, unsigned* cnt,
+                                                   unsigned (&bk)[kOrderPer]) {
+    const int t = threadIdx.x, copy = t % kOrderCopies, base = blockIdx.x * kOrderChunk + t;
+    for (int e = t; e < kOrderBuckets * kOrderCopies; e += kOrderBlock) cnt[e] = 0u;
+    unsigned v[kOrderPer];
 #pragma unroll
-        for (int k = 0; k < kB; k++) {
-            const int i = base + k * kOrderThreads;
-            v[k] = i < n ? cost[i] : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < kB; k++) m = max(m, v[k]);
+    for (int k = 0; k < kOrderPer; k++) {
+        const int i = base + k * kOrderBlock;
+        v[k] = i < n ? cost[i] : 0u;
     }
-    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
-    if (lane == 0) part[wave] = m;
-    for (int e = t; e < kOrderBuckets * kCopies; e += kOrderThreads) cnt[e] = 0u;
+#pragma unroll
+    for (int k = 0; k < kOrderPer; k++) bk[k] = cost_bucket(v[k]);
     __syncthreads();
-    m = 0;
-    for (int w = 0; w < kOrderThreads / 64; w++) m = max(m, part[w]);
-    // bucket = c (kOrderBuckets - 1) / (max + 1) in 32.32 fixed point for max >= kOrderBuckets - 1,
-    // where the factor is below 2^32 and c <= max keeps the bucket below kOrderBuckets - 1; smaller
-    // costs are their own buckets (the factor would not fit 32 bits)
-    const bool direct = m < (unsigned)(kOrderBuckets - 1);
-    const unsigned scale =
-        direct ? 0u : (unsigned)(((unsigned long long)(kOrderBuckets - 1) << 32) / ((unsigned long long)m + 1ull));
-    auto bucket = [&](unsigned v) { return direct ? v : __umulhi(v, scale); };
-    for (int base = t; base < n; base += kOrderThreads * kB) {
-        unsigned v[kB];
 #pragma unroll
-        for (int k = 0; k < kB; k++) {
-            const int i = base + k * kOrderThreads;
-            v[k] = i < n ? cost[i] : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < kB; k++)
-            if (base + k * kOrderThreads < n) atomicAdd(&cnt[bucket(v[k]) * kCopies + copy], 1u);
-    }
+    for (int k = 0; k < kOrderPer; k++)
+        if (base + k * kOrderBlock < n) atomicAdd(&cnt[bk[k] * kOrderCopies + copy], 1u);
     __syncthreads();
-    // exclusive scan over (bucket, copy) from the top bucket down: thread t owns the kCopies entries of
-    // bucket kOrderBuckets - 1 - t; their local prefix, an inclusive wave scan of the totals, then the
-    // wave totals.
-    const int b = kOrderBuckets - 1 - t;
-    unsigned loc[kCopies], mine = 0;
+}
+
+__global__ __launch_bounds__(kOrderBlock) void order_count_kernel(unsigned* __restrict__ sched, int n) {
+    __shared__ unsigned cnt[kOrderBuckets * kOrderCopies];
+    unsigned bk[kOrderPer];
+    order_block_counts(sched + mm::kSchedHeader, n, cnt, bk);
+    for (int b = threadIdx.x; b < kOrderBuckets; b += kOrderBlock) {
+        unsigned s = 0;
 #pragma unroll
-    for (int r = 0; r < kCopies; r++) {
-        loc[r] = mine;
-        mine += cnt[b * kCopies + r];
+        for (int r = 0; r < kOrderCopies; r++) s += cnt[b * kOrderCopies + r];
+        if (s) atomicAdd(&sched[mm::kSchedCount + b], s);
     }
+}
+
+// One workgroup of kOrderBuckets threads: thread t takes bucket kOrderBuckets - 1 - t; an inclusive
+// wave scan of the counts, then the wave totals, give each bucket's first slot.
+__global__ __launch_bounds__(kOrderBuckets) void order_scan_kernel(unsigned* __restrict__ sched) {
+    __shared__ unsigned part[kOrderBuckets / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, b = kOrderBuckets - 1 - t;
+    const unsigned mine = sched[mm::kSchedCount + b];
     unsigned incl = mine;
     for (int o = 1; o < 64; o <<= 1) {
         const unsigned v = (unsigned)__shfl_up((int)incl, o);
         if (lane >= o) incl += v;
     }
-    __syncthreads();  // every cnt read above is done before cnt is rewritten below
     if (lane == 63) part[wave] = incl;
     __syncthreads();
     unsigned before = 0;
     for (int w = 0; w < wave; w++) before += part[w];
-    const unsigned first = before + incl - mine;  // the bucket's first slot
+    sched[mm::kSchedCursor + b] = before + incl - mine;
+}
+
+__global__ __launch_bounds__(kOrderBlock) void order_place_kernel(unsigned* __restrict__ sched, int n,
+                                                                   int* __restrict__ order) {
+    __shared__ unsigned cnt[kOrderBuckets * kOrderCopies];
+    unsigned bk[kOrderPer];
+    order_block_counts(sched + mm::kSchedHeader, n, cnt, bk);
+    // the block's run of slots per bucket, split among the copies
+    for (int b = threadIdx.x; b < kOrderBuckets; b += kOrderBlock) {
+        unsigned loc[kOrderCopies], s = 0;
 #pragma unroll
-    for (int r = 0; r < kCopies; r++) cnt[b * kCopies + r] = first + loc[r];
-    __syncthreads();
-    for (int base = t; base < n; base += kOrderThreads * kB) {
-        unsigned v[kB];
-#pragma unroll
-        for (int k = 0; k < kB; k++) {
-            const int i = base + k * kOrderThreads;
-            v[k] = i < n ? cost[i] : 0u;
+        for (int r = 0; r < kOrderCopies; r++) {
+            loc[r] = s;
+            s += cnt[b * kOrderCopies + r];
         }
+        const unsigned first = s ? atomicAdd(&sched[mm::kSchedCursor + b], s) : 0u;
 #pragma unroll
-        for (int k = 0; k < kB; k++) {
-            const int i = base + k * kOrderThreads;
-            if (i < n) order[atomicAdd(&cnt[bucket(v[k]) * kCopies + copy], 1u)] = i;
+        for (int r = 0; r < kOrderCopies; r++) cnt[b * kOrderCopies + r] = first + loc[r];
+    }
+    __syncthreads();
+    const int t = threadIdx.x, copy = t % kOrderCopies, base = blockIdx.x * kOrderChunk + t;
+#pragma unroll
+    for (int k = 0; k < kOrderPer; k++) {
+        const int i = base + k * kOrderBlock;
+        if (i < n) {
+            const unsigned slot = atomicAdd(&cnt[bk[k] * kOrderCopies + copy], 1u);
+            if (slot < (unsigned)n) order[slot] = i;  // always, with counts from this record
         }
     }
 }
@@ -1781,7 +1806,7 @@ int launch_shadow(int fset, const ShadowConsts& c, const TexView& weather, const
 
 int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, bool boundedSteps, const ViewDesc* views,
                  int nViews, const MarchTextures& t, int total, unsigned long long* counts, const int* order,
-                 unsigned* cost, void* stream) {
+                 unsigned* sched, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     const bool cnt = counts != nullptr;
     const bool bs = boundedSteps;
@@ -1791,14 +1816,20 @@ int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, bool bo
     if (!k) return MM_E_FORMAT;
     const unsigned grid = (unsigned)total;
     void* args[] = {(void*)&views, (void*)&nViews, (void*)&t, (void*)&total, (void*)&counts, (void*)&order,
-                    (void*)&cost};
+                    (void*)&sched};
     if (hipLaunchKernel(k, dim3(grid), dim3(64), args, 0, s) != hipSuccess) return MM_E_DEVICE;
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 
-int launch_tile_order(const unsigned* cost, int n, int* order, void* stream) {
+int launch_tile_order(unsigned* sched, int n, int* order, void* stream) {
     if (n <= 0) return MM_OK;
-    hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, (hipStream_t)stream, cost, n, order);
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((unsigned)((n + kOrderChunk - 1) / kOrderChunk));
+    hipLaunchKernelGGL(order_count_kernel, grid, dim3(kOrderBlock), 0, s, sched, n);
+    if (hipGetLastError() != hipSuccess) return MM_E_DEVICE;
+    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(kOrderBuckets), 0, s, sched);
+    if (hipGetLastError() != hipSuccess) return MM_E_DEVICE;
+    hipLaunchKernelGGL(order_place_kernel, grid, dim3(kOrderBlock), 0, s, sched, n, order);
     return hipGetLastError() == hipSuccess ? MM_OK : MM_E_DEVICE;
 }
 

@@ -55,10 +55,16 @@ struct MarchConsts {
     unsigned* collectCount;
     unsigned collectCap, collectKeep, collectSeed;
     // tile schedule (mm_march_clouds_scheduled), both null otherwise: the physical tile each block
-    // marches (a permutation of 0 .. tiles - 1) and each physical tile's cost for the next launch
+    // marches (a permutation of 0 .. tiles - 1) and the schedule record (kSchedHeader words of order
+    // state, then each physical tile's cost for the next launch)
     const int* tileOrder;
-    unsigned* tileCost;
+    unsigned* tileSched;
 };
+
+// A tile schedule's device record: kOrderBuckets per-bucket tile counts, kOrderBuckets per-bucket
+// next slots, then one cost per tile (from kSchedHeader on).
+constexpr int kOrderBuckets = 1024;
+constexpr int kSchedCount = 0, kSchedCursor = kOrderBuckets, kSchedHeader = 2 * kOrderBuckets;
 
 // Device view of one texture.
 struct TexView {
@@ -134,11 +140,11 @@ inline bool bounded_steps(uint32_t flags, int maxSteps, int maxCoarse, int maxFi
     return (flags & MM_MARCH_BOUNDED) != 0 && maxSteps < 0 && maxCoarse >= 0 && maxFine >= 0;
 }
 
-// order: null (launch order) or a permutation of the plan's global tiles; cost: null or one slot per
-// global tile for the next launch's order.
+// order: null (launch order) or a permutation of the plan's global tiles; sched: null or the plan's
+// schedule record (kSchedHeader words, then one cost slot per global tile for the next launch's order).
 int launch_views(int fset, int mode, int lanesPerRay, bool floatWeights, bool boundedSteps, const ViewDesc* views,
                  int nViews, const MarchTextures& t, int total, unsigned long long* counts, const int* order,
-                 unsigned* cost, void* stream);
+                 unsigned* sched, void* stream);
 
 size_t surrogate_model_bytes();
 int surrogate_init(void* model, uint32_t seed, void* stream);
@@ -156,9 +162,10 @@ const uint4* surrogate_infer_frags(const void* model);  // FP16 inference fragme
 
 int launch_march(int fset, const MarchConsts& c, const MarchTextures& t, float* out, unsigned long long* counts,
                  void* stream);
-// order <- the n physical tiles sorted by cost, largest first (bucketed: 1024 cost buckets), in one
-// workgroup on `stream`.
-int launch_tile_order(const unsigned* cost, int n, int* order, void* stream);
+// order <- the n physical tiles of a schedule record sorted by cost, largest first (bucketed: 1024 cost
+// buckets), in three launches on `stream`. The record's counts must be clear (every scheduled march
+// clears them for the next order).
+int launch_tile_order(unsigned* sched, int n, int* order, void* stream);
 
 // Noise generation launchers.
 // allow_side = false keeps every generator on `stream` (no second stream is created or used)
