@@ -104,6 +104,15 @@ def test_config4_graph_launch_equals_single_view_marches(mm, views64, trained, c
         for i, (a, b) in enumerate(zip(vs.outs, ref)):
             assert torch.equal(a, b), (cone, rep, i, vs.specs[i])
     ms = multiview.time_launch(lambda: g.replay(), reps=5)
+    # eager launches after recorded ones march all views' tiles longest first (the tile order over
+    # the whole set, 0.5 M tiles): the same images
+    for rep in range(2):
+        for o in vs.outs:
+            o.fill_(-1.0)
+        plan.launch()
+        torch.cuda.synchronize()
+        for i, (a, b) in enumerate(zip(vs.outs, ref)):
+            assert torch.equal(a, b), (cone, "ordered", rep, i, vs.specs[i])
     cnt = torch.zeros(4, dtype=torch.int64, device="cuda")
     plan.launch(cnt)
     torch.cuda.synchronize()

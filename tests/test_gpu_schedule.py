@@ -62,6 +62,21 @@ def test_scheduled_launches_equal_the_plain_march(mm, gpu, case):
     sched.close()
 
 
+def test_scheduled_1080p_frames_equal_the_plain_march(mm, gpu):
+    """The headline shape: 32 400 tiles, so every order launch runs several workgroups."""
+    W, H = 1920, 1080
+    ns = _noise(mm)
+    prm = mm.march_params("scenario")
+    sched = mm.MarchSchedule(1 << 16)
+    for rep, (sun_deg, t) in enumerate([(25.0, 0.0), (25.0, 0.0), (25.0, 0.0), (40.0, 120.0)]):
+        ubos = _scene(mm, W, H, sun_deg, t)
+        ref, cref = _march(mm, ns, ubos, W, H, prm)
+        got, cgot = _march(mm, ns, ubos, W, H, prm, schedule=sched)
+        assert np.array_equal(got, ref), rep
+        assert (cgot == cref).all(), (rep, cgot, cref)
+    sched.close()
+
+
 def test_schedule_follows_shape_changes(mm, gpu):
     """A launch of another shape marches in the default order and starts a new record; going back
     to the first shape then uses that shape's costs again after one launch."""
