@@ -262,28 +262,36 @@ def test_tile_row_order_for_camera_orientations(mm, gpu, orc, tex, pitch):
 @pytest.mark.parametrize("W,H", [(1, 1), (3, 2), (9, 1), (1, 17), (13, 5)])
 def test_tiny_and_ragged_images_for_every_kernel_shape(mm, gpu, tex, W, H):
     """Tiles cut by the image edge in both directions, for every lanes-per-ray value and tile width,
-    full frame and one phase: the same image as the lock-step kernel, bit for bit, and nothing
-    written beyond the image width (three pad pixels per row keep their fill)."""
+    full frame and one phase, plain and through a tile schedule: the same image as the lock-step
+    kernel, bit for bit, and nothing written beyond the image width (three pad pixels per row keep
+    their fill)."""
     import torch
 
     ns = upload(mm, tex, "fast")
     cam, sun, sky = mm.build_ubos(mm.scenario_scene(aspect=W / H, pitch_deg=10.0))
     P = W + 3
 
-    def run(params, phase):
+    def run(params, phase, schedule=None):
         out = torch.full((H, P, 4), -3.0, dtype=torch.float32, device="cuda")
-        mm.march_clouds(cam, sun, sky, ns, out, W, H, phase=phase, params=params, pitch=P)
+        mm.march_clouds(cam, sun, sky, ns, out, W, H, phase=phase, params=params, pitch=P, schedule=schedule)
         torch.cuda.synchronize()
         return out.cpu().numpy()
 
     base = mm.march_params("scenario")
     base.flags |= mm.MARCH_NO_BATCH
+    sched = mm.MarchSchedule(64)
     for phase in (-1, 0, 7):
         ref = run(base, phase)
         assert (ref[:, W:] == -3.0).all()
         for lanes, tw in ((1, 8), (1, 32), (2, 8), (4, 8), (8, 8)):
             g = run(mm.march_params("scenario", lanes_per_ray=lanes, tile_width=tw), phase)
             assert np.array_equal(g, ref), (W, H, phase, lanes, tw)
+            # the same shape through a tile schedule, twice (the second launch ordered): a handful of
+            # tiles, down to one
+            for rep in range(2):
+                g = run(mm.march_params("scenario", lanes_per_ray=lanes, tile_width=tw), phase, sched)
+                assert np.array_equal(g, ref), (W, H, phase, lanes, tw, "scheduled", rep)
+    sched.close()
 
 
 def test_compute_shader_operator_ping_pong(mm, gpu, tex):
