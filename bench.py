@@ -264,6 +264,16 @@ def post_leg(mm, cam, sun, out, W, H, steps, warmup):
             "pass_gbs": round(moved / (ms * 1e-3) / 1e9, 1)}
 
 
+def noise_block(ns, noise_ms):
+    """The four generators' position against HBM: the bytes of the four textures they write per
+    step over the step's generation time (their bound is VALU issue and LDS reads; their writes are
+    a few MiB)."""
+    written = sum(int(t.desc.bytes) for t in (ns.weather, ns.curl, ns.low_res, ns.hi_res))
+    gbs = written / (noise_ms * 1e-3) / 1e9
+    return {"ms": round(noise_ms, 4), "bytes_written": written, "write_gbs": round(gbs, 1),
+            "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def march_kernel_name(args):
     """The kernel a full-frame plain march launches: lock-step, cone-batched (1 lane per ray: the
     default of a scheduled march) or ray-parallel (2 lanes per ray: the unscheduled default)."""
@@ -532,6 +542,7 @@ def main():
         "march_ms_unscheduled": unscheduled,
         "median_ms": {"frame": round(median(step_ms_sorted), 4), "march": round(median(march_ms_sorted), 4)},
         "noise_ms": round(noise_ms, 4),
+        "noise": noise_block(ns, noise_ms),
         "density_samples_per_frame": n_density,
         "detail_samples_per_frame": n_detail,
         "cone_jobs_per_frame": n_jobs,
