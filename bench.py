@@ -2,7 +2,8 @@
 
 One step = one frame of the 1920x1080 scenario workload: every noise texture regenerated on the GPU
 (weather 512^2, curl 128^2, low-res 128^3, hi-res 32^3), then the full-frame cloud march (all
-pixels). Inputs live in HBM before timing starts. Prints one JSON line with ms/frame, density
+pixels; its tiles longest first by the previous step's tile costs unless --no-schedule). Inputs live
+in HBM before timing starts. Prints one JSON line with ms/frame, density
 samples/s (the value), the march kernel's roofline position and a CPU baseline (the FP32 oracle
 on this host's cores, on a bounded sample of the same frame).
 """
