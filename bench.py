@@ -378,7 +378,8 @@ def sweep_leg(mm, args, prm, W, H):
             est = valu_live(dict(vp, insts_per_launch=vp["insts_per_launch"] * (nd + nh) / vp["samples_per_launch"]), march)
             row["roofline_valu_est"] = {"insts_per_launch": round(est["insts_per_launch"]), "frac": est["frac"]}
         rows.append(row)
-        print(json.dumps({"sweep_scene": row}), flush=True)
+        # per-scene progress on stderr: stdout carries the one result line
+        print(json.dumps({"sweep_scene": row}), file=sys.stderr, flush=True)
     ms = sorted(r["ms_per_frame"] for r in rows)
     sps = sorted(r["density_samples_per_s"] for r in rows)
     fr = sorted(r["roofline_hbm"]["frac"] for r in rows)
