@@ -410,9 +410,11 @@ int mm_load_volume_rgba8(const char* prefix, int32_t depth, uint8_t* rgba, size_
  * 1061-1071): many views of any resolution marched by one launch over the tiles of all views (one
  * wave each), handed to the CUs by the hardware dispatcher as slots free up. Views share the curl
  * and noise volumes and bring their own uniforms, weather map and output image. Each view's image is identical to mm_march_clouds of that view with the same
- * params. A plan holds the per-view constants in device memory; mm_march_views enqueues one kernel
- * (no allocation, no host transfer, no memset), so it can be captured into a HIP graph and
- * replayed. */
+ * params. A plan holds the per-view constants in device memory; mm_march_views enqueues the march
+ * (after the plan's first launch, preceded by three small launches that order all views' tiles
+ * longest first by the previous launch's tile costs; the images do not depend on the order) with
+ * no allocation, no host transfer and no memset, so it can be captured into a HIP graph and
+ * replayed (a capture of the plan's first launch keeps the launch order). */
 typedef struct mm_view {
     mm_camera_ubo camera;
     mm_sun_ubo sun;
