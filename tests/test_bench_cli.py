@@ -50,3 +50,14 @@ def test_traffic_comes_from_the_committed_summary():
     assert b.traffic_from_profiles("fast", 1920, 1080, "march_kernel") is None
     assert b.valu_from_profiles("fast", 1920, 1080, "march_kernel") is None
     assert b.valu_from_profiles("fast", 1920, 1080, d["kernel"].split("<")[0])["insts_per_launch"] > 0
+
+
+def test_noise_block_counts_the_written_textures():
+    """The generators' HBM position: the four textures' bytes over the generation time."""
+    b = _bench()
+    tex = lambda n: type("T", (), {"desc": type("D", (), {"bytes": n})})
+    ns = type("N", (), {"weather": tex(1000), "curl": tex(2000), "low_res": tex(3000), "hi_res": tex(4000)})
+    blk = b.noise_block(ns, 0.5)
+    assert blk["bytes_written"] == 10000
+    assert abs(blk["write_gbs"] - 10000 / 0.5e-3 / 1e9) < 0.05
+    assert blk["frac_of_hbm_peak"] == round(blk["write_gbs"] / b.HBM_PEAK_GBS, 4) or blk["write_gbs"] < 1
